@@ -1,0 +1,224 @@
+"""ctypes bindings of libcsg.so (C-ABI of the HIP geodesic force path) and libcsg_host.so (C++ host layer).
+
+Both libraries are built in-tree by curvedspacesim_amd/build.py. Loading fails loudly when they are missing: there is
+no Python or CPU fallback for anything in this package.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+_csg = None
+_host = None
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+VP = C.c_void_p
+
+
+class CsgError(RuntimeError):
+    pass
+
+
+class csg_potential(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("p0", C.c_double), ("p1", C.c_double), ("p2", C.c_double)]
+
+
+HARMONIC, GAUSSIAN, TRUNC_GAUSSIAN, NO_POTENTIAL = 0, 1, 2, 3
+PAIR_UNREACHED, PAIR_BEYOND, PAIR_SENTINEL = 1, 4, 16
+
+
+def _path(name):
+    p = os.path.join(PKG, name)
+    if not os.path.exists(p):
+        raise CsgError(f"{name} is not built; run curvedspacesim_amd/build.py (or __graft_entry__.build())")
+    return p
+
+
+def csg():
+    """libcsg.so with argument types declared."""
+    global _csg
+    if _csg is not None:
+        return _csg
+    L = C.CDLL(_path("libcsg.so"))
+    L.csg_ctx_create.argtypes = [C.POINTER(VP), C.c_int]
+    L.csg_ctx_destroy.argtypes = [VP]
+    L.csg_last_error.argtypes = [VP]
+    L.csg_last_error.restype = C.c_char_p
+    L.csg_device_name.argtypes = [VP]
+    L.csg_device_name.restype = C.c_char_p
+    L.csg_mesh_upload.argtypes = [VP, C.c_int32, _f64p, C.c_int32, _i32p]
+    L.csg_mesh_info.argtypes = [VP, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_double), _f64p, _f64p]
+    L.csg_set_submeshing.argtypes = [VP, C.c_int32, C.c_double]
+    L.csg_distance_batch.argtypes = [VP, C.c_int32, _i32p, _f64p, _i32p, _i32p, _f64p, VP, VP, VP, VP, VP]
+    L.csg_pair_forces.argtypes = [VP, C.c_int32, _i32p, _f64p, C.c_int32, VP, _i32p, _i32p, VP, C.c_double,
+                                  C.POINTER(csg_potential), C.c_int32, C.c_double, VP, VP, VP, VP, VP, VP]
+    L.csg_transport_batch.argtypes = [VP, C.c_int32, _i32p, _f64p, _f64p, C.c_int32, VP, _i32p]
+    L.csg_positions.argtypes = [VP, C.c_int32, _i32p, _f64p, _f64p]
+    _csg = L
+    return L
+
+
+def host():
+    """libcsg_host.so with argument types declared."""
+    global _host
+    if _host is not None:
+        return _host
+    csg()  # dependency
+    L = C.CDLL(_path("libcsg_host.so"))
+    L.csgh_last_error.restype = C.c_char_p
+    L.csgh_mesh_icosphere.argtypes = [C.c_int, C.c_double, C.POINTER(VP)]
+    L.csgh_mesh_torus.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_uint32, C.POINTER(VP)]
+    L.csgh_mesh_bumpy_sphere.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint32, C.POINTER(VP)]
+    L.csgh_mesh_read_off.argtypes = [C.c_char_p, C.POINTER(VP)]
+    L.csgh_mesh_from_arrays.argtypes = [C.c_int, _f64p, C.c_int, _i32p, C.POINTER(VP)]
+    L.csgh_mesh_size.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.csgh_mesh_copy.argtypes = [VP, _f64p, _i32p]
+    L.csgh_mesh_free.argtypes = [VP]
+    _host = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(VP)
+
+
+def _hmesh(fn, *args):
+    L = host()
+    h = VP()
+    rc = getattr(L, fn)(*args, C.byref(h))
+    if rc != 0:
+        raise CsgError(L.csgh_last_error().decode())
+    nv, nf = C.c_int(), C.c_int()
+    L.csgh_mesh_size(h, C.byref(nv), C.byref(nf))
+    xyz = np.zeros(3 * nv.value)
+    tri = np.zeros(3 * nf.value, np.int32)
+    L.csgh_mesh_copy(h, xyz, tri)
+    L.csgh_mesh_free(h)
+    return xyz.reshape(-1, 3), tri.reshape(-1, 3)
+
+
+def icosphere(level, radius=1.0):
+    return _hmesh("csgh_mesh_icosphere", int(level), float(radius))
+
+
+def jittered_torus(R=3.0, r=1.0, nu=200, nv=100, jitter=0.02, seed=13377):
+    return _hmesh("csgh_mesh_torus", float(R), float(r), int(nu), int(nv), float(jitter), int(seed))
+
+
+def bumpy_sphere(level=6, n_bumps=20, amplitude=0.15, width=0.3, seed=13377):
+    return _hmesh("csgh_mesh_bumpy_sphere", int(level), int(n_bumps), float(amplitude), float(width), int(seed))
+
+
+def read_off(path):
+    return _hmesh("csgh_mesh_read_off", path.encode())
+
+
+class Context:
+    """One device context of libcsg.so with a mesh."""
+
+    def __init__(self, device=0):
+        self.L = csg()
+        h = VP()
+        rc = self.L.csg_ctx_create(C.byref(h), int(device))
+        if rc != 0:
+            raise CsgError(f"csg_ctx_create failed ({rc}): no gfx950 device {device} or HIP runtime error")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.csg_ctx_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise CsgError(f"{what} failed ({rc}): {self.L.csg_last_error(self.h).decode()}")
+
+    @property
+    def device_name(self):
+        return self.L.csg_device_name(self.h).decode()
+
+    def upload_mesh(self, xyz, tri):
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        tri = np.ascontiguousarray(tri, np.int32).reshape(-1, 3)
+        self._check(self.L.csg_mesh_upload(self.h, len(xyz), xyz.ravel(), len(tri), tri.ravel()), "csg_mesh_upload")
+        self.nV, self.nF = len(xyz), len(tri)
+
+    def mesh_info(self):
+        nv, nf, area = C.c_int32(), C.c_int32(), C.c_double()
+        lo, hi = np.zeros(3), np.zeros(3)
+        self._check(self.L.csg_mesh_info(self.h, C.byref(nv), C.byref(nf), C.byref(area), lo, hi), "csg_mesh_info")
+        return dict(nV=nv.value, nF=nf.value, area=area.value, lo=lo, hi=hi)
+
+    def set_submeshing(self, enabled, maximum_distance=1.0):
+        self._check(self.L.csg_set_submeshing(self.h, int(bool(enabled)), float(maximum_distance)), "csg_set_submeshing")
+
+    def distance_batch(self, src_face, src_bary, tgt_offsets, tgt_face, tgt_bary, threshold=None):
+        src_face = np.ascontiguousarray(src_face, np.int32)
+        n = len(src_face)
+        off = np.ascontiguousarray(tgt_offsets, np.int32)
+        P = int(off[-1]) if n else 0
+        d = np.zeros(P)
+        st = np.zeros((P, 3))
+        et = np.zeros((P, 3))
+        fl = np.zeros(P, np.uint32)
+        thr = None if threshold is None else np.ascontiguousarray(threshold, np.float64)
+        self._check(self.L.csg_distance_batch(self.h, n, src_face, np.ascontiguousarray(src_bary, np.float64).ravel(), off,
+                                              np.ascontiguousarray(tgt_face, np.int32),
+                                              np.ascontiguousarray(tgt_bary, np.float64).ravel(), _ptr(thr), _ptr(d),
+                                              _ptr(st), _ptr(et), _ptr(fl)), "csg_distance_batch")
+        return d, st, et, fl
+
+    def pair_forces(self, face, bary, tgt_offsets, tgt_idx, src_idx=None, patch_radius=None, rmax=np.inf,
+                    potential=None, use_sentinel=False, sentinel=0.0):
+        face = np.ascontiguousarray(face, np.int32)
+        bary = np.ascontiguousarray(bary, np.float64).reshape(-1, 3)
+        off = np.ascontiguousarray(tgt_offsets, np.int32)
+        nsrc = len(off) - 1
+        idx = np.ascontiguousarray(tgt_idx, np.int32)
+        P = int(off[-1])
+        sidx = None if src_idx is None else np.ascontiguousarray(src_idx, np.int32)
+        rad = None if patch_radius is None else np.ascontiguousarray(patch_radius, np.float64)
+        pot = None
+        if potential is not None:
+            pot = csg_potential(int(potential[0]), *[float(x) for x in (list(potential[1:]) + [0, 0, 0])[:3]])
+        d = np.zeros(P)
+        st = np.zeros((P, 3), np.float32)
+        et = np.zeros((P, 3), np.float32)
+        fl = np.zeros(P, np.uint32)
+        F = np.zeros((nsrc, 3))
+        E = np.zeros(nsrc)
+        rm = float(rmax) if np.isfinite(rmax) else 1e300
+        self._check(self.L.csg_pair_forces(self.h, len(face), face, bary.ravel(), nsrc, _ptr(sidx), off, idx, _ptr(rad), rm,
+                                           C.byref(pot) if pot is not None else None, int(bool(use_sentinel)),
+                                           float(sentinel), _ptr(d), _ptr(st), _ptr(et), _ptr(fl), _ptr(F), _ptr(E)),
+                    "csg_pair_forces")
+        return dict(dist=d, start=st, end=et, flags=fl, forces=F, energy=E)
+
+    def transport_batch(self, face, bary, disp, vecs=None):
+        face = np.array(face, np.int32).ravel().copy()
+        n = len(face)
+        bary = np.array(bary, np.float64).reshape(n, 3).copy()
+        disp = np.array(disp, np.float64).reshape(n, 3).copy()
+        nvec = 0
+        if vecs is not None:
+            vecs = np.array(vecs, np.float64).reshape(n, -1, 3).copy()
+            nvec = vecs.shape[1]
+        status = np.zeros(n, np.int32)
+        rc = self.L.csg_transport_batch(self.h, n, face, bary.ravel(), disp.ravel(), nvec,
+                                        _ptr(vecs) if vecs is not None else None, status)
+        if rc not in (0, -5):
+            self._check(rc, "csg_transport_batch")
+        return face, bary, disp, vecs, status
+
+    def positions(self, face, bary):
+        face = np.ascontiguousarray(face, np.int32)
+        out = np.zeros((len(face), 3))
+        self._check(self.L.csg_positions(self.h, len(face), face, np.ascontiguousarray(bary, np.float64).ravel(),
+                                         out.ravel()), "csg_positions")
+        return out

@@ -1,0 +1,53 @@
+"""Builds the native libraries in-tree: libcsg.so (HIP kernels + C-ABI, gfx950) and libcsg_host.so (C++ host layer)."""
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(PKG, "csrc")
+ROOT = os.path.dirname(PKG)
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_SOURCES = ["csg_api.hip", "geodesic_kernel.hip", "transport_kernel.hip"]
+HOST_SOURCES = ["host/meshGenerators.cpp", "host/hostCapi.cpp"]
+# packed FP32 instructions are kept out of every code object (see the hipcc feature flag below)
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _run(cmd, cwd):
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed: " + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r
+
+
+def build_csg():
+    out = os.path.join(PKG, "libcsg.so")
+    cmd = [HIPCC] + HIP_FLAGS + [os.path.join(CSRC, s) for s in HIP_SOURCES] + ["-o", out + ".tmp"]
+    _run(cmd, CSRC)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_host():
+    out = os.path.join(PKG, "libcsg_host.so")
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall",
+           "-I" + os.path.join(ROOT, "include")] + [os.path.join(CSRC, s) for s in HOST_SOURCES] + \
+          ["-L" + PKG, "-lcsg", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+    _run(cmd, CSRC)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_all(verbose=False):
+    a = build_csg()
+    b = build_host()
+    if verbose:
+        print("built", a, b)
+    return a, b
+
+
+if __name__ == "__main__":
+    build_all(verbose=True)
+    sys.exit(0)

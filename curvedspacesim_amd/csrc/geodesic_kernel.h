@@ -1,0 +1,50 @@
+// Launch interface of the batched geodesic + pair-force kernels (geodesic_kernel.hip).
+#pragma once
+
+#include "csg_common.h"
+
+namespace csg {
+
+// LDS capacities of the fast variant: face hash slots (at most half filled), vertex hash slots (at most 3/4
+// filled), live window FIFO entries, targets per source. A source that exceeds any of them is solved again by the
+// scratch variant.
+constexpr int kLdsFaceCap = 1024;
+constexpr int kLdsVertCap = 512;
+constexpr int kLdsQueueCap = 256;
+constexpr int kLdsTargetCap = 256;
+
+struct GeoParams {
+    DeviceMesh mesh;
+    const int32_t* __restrict__ pface;  // particle faces
+    const double* __restrict__ pbary;   // particle barycentric coordinates, 3 per particle
+    int nSrc;
+    const int32_t* __restrict__ srcIdx;  // particle index of each source (nullptr: source i is particle i)
+    const int32_t* __restrict__ toff;    // nSrc + 1 offsets into tidx and into the pair outputs
+    const int32_t* __restrict__ tidx;    // particle index of each target
+    const double* __restrict__ patchRadius;  // per source; nullptr: whole mesh (no submeshing)
+    double rMax;                         // windows beyond this lower-bound distance are not propagated
+    // outputs, all optional
+    double* pairDist;
+    float* pairStart;   // 3 per pair
+    float* pairEnd;     // 3 per pair
+    uint32_t* pairFlags;
+    double* forces;     // 3 per source
+    double* energy;     // per source
+    int* status;        // per source: 0 solved, 1 capacity exceeded
+    Potential pot;
+    int useSentinel;    // unreachable pairs get (sentinelDist, (0,0,1)) as the reference does
+    double sentinelDist;
+};
+
+struct ScratchLayout {
+    void* base;
+    size_t bytesPerBlock;
+    int fcap, vcap, qcap, tcap;  // powers of two (tcap any)
+};
+
+size_t scratchBytesPerBlock(int fcap, int vcap, int qcap, int tcap);
+hipError_t launchGeodesicLds(const GeoParams& P, hipStream_t stream);
+hipError_t launchGeodesicScratch(const GeoParams& P, const ScratchLayout& L, const int32_t* list, int nList, int blocks,
+                                 hipStream_t stream);
+
+}  // namespace csg

@@ -1,0 +1,870 @@
+// Batched exact polyhedral geodesics + pair forces on gfx950.
+//
+// One workgroup (one 64-wide wavefront) solves one source particle:
+//   1. patch   : the reference's submesh rule (src/utility/submesher.cpp:55-147) as a breadth-first search over
+//                faces into an LDS hash table (global face id -> slot); every slot caches its corners' vertex slots and
+//                its neighbours' face slots, so the propagation touches HBM only for face geometry and coordinates.
+//   2. windows : exact window (interval) propagation in FP64 (the method behind CGAL's Surface_mesh_shortest_path,
+//                src/utility/meshUtilities.h:44-48), up to 64 work items per step from an LDS FIFO, one per lane.
+//                Vertex distance bounds are LDS 64-bit atomic minima on the IEEE bit pattern (distances are
+//                non-negative), and a window is dropped when those bounds show it cannot carry a shortest path
+//                anywhere on its interval (Xin & Wang's filter). Saddle vertices (angle sum >= 2 pi) and vertices on
+//                the patch border are pseudo-sources: reaching one with a smaller distance queues a vertex event
+//                that opens a window on the far edge of each incident patch face.
+//   3. targets : every window is tested against the targets of the face it enters; a target keeps the minimum
+//                (atomic on the bit pattern) and, in a second pass, the tangents of the path that set it. The start
+//                tangent is carried per window as a rotation from the window's edge frame to the source-face frame,
+//                so no path is traced back (reference: computePathDistanceAndTangents,
+//                src/utility/meshUtilities.cpp:360-380).
+//   4. forces  : the pair force (src/forces/baseForce.cpp:12-28) summed per source with a wavefront reduction.
+// A source whose patch, vertex set, target list or live FIFO exceeds the LDS capacities is flagged and solved again
+// by the same code with its tables in a per-workgroup HBM scratch slice; that variant also serves the whole-mesh
+// mode of triangulatedMeshSpace::distance (src/models/triangulatedMeshSpace.cpp:222-237).
+#include <math.h>
+
+#include "csg_common.h"
+#include "geodesic_kernel.h"
+
+namespace csg {
+
+namespace {
+
+constexpr double kInf = __builtin_huge_val();
+constexpr double kFilterTol = 1e-12;  // relative slack before a window is declared useless
+constexpr double kEdgeTol = 1e-12;    // relative distance at which an interval end counts as the edge's endpoint
+constexpr int kMaxSteps = 1 << 18;    // FIFO steps per source before it is abandoned as over capacity
+
+__device__ inline unsigned long long dbits(double d) { return (unsigned long long)__double_as_longlong(d); }
+__device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
+__device__ inline uint32_t hashOf(uint32_t key) { return (key * 2654435761u) >> 7; }
+
+// Tables of one source. LDS variant: every pointer refers to __shared__ storage; scratch variant: to a slice of a
+// global buffer. Hash capacities are powers of two.
+template <typename IT>
+struct Tables {
+    int32_t* fkey;   // face hash: global face id or -1
+    IT* fcv;         // 3 per face slot: corner vertex slots
+    IT* fnb;         // 3 per face slot: neighbour face slot * 4 + its local edge index, or -1 outside the patch
+    IT* fhead;       // per face slot: first target lying in that face, or -1
+    int32_t* vkey;   // vertex hash: global vertex id or -1
+    unsigned long long* vdist;
+    float2* vtan;    // start tangent of the best path to the vertex (source-face frame)
+    int32_t* vflg;   // 1: may act as a pseudo-source
+    // FIFO of work items: windows and vertex events
+    double *qb0, *qb1, *qsx, *qsy, *qsg;
+    int32_t* qfk;    // window: face slot * 8 + k * 2 + rooted; vertex event: -(vertex slot) - 1
+    float2* qt;      // rooted window: rotation (cos, sin) to the source-face frame; otherwise its fixed start tangent
+    // targets of the source
+    IT* tnext;
+    unsigned long long* tbest;
+    float2* tst;     // start tangent (source-face frame)
+    float4* tet;     // end tangent (xyz)
+    int fmask, vmask, qcap, tcap;
+    int flimit, vlimit;
+};
+
+struct Shared {
+    int head, tail, nFaces, nVerts, overflow, nCur, nNext, goals;
+    double S[3], t1[3], t2[3];
+};
+
+template <class TT>
+__device__ int faceLookup(const TT& T, int g) {
+    uint32_t h = hashOf((uint32_t)g) & T.fmask;
+    for (int p = 0; p <= T.fmask; ++p) {
+        int k = T.fkey[h];
+        if (k == g) return (int)h;
+        if (k == -1) return -1;
+        h = (h + 1) & T.fmask;
+    }
+    return -1;
+}
+
+// slot of g (inserting it if needed), or -1 past the fill limit; `fresh` reports whether this call inserted it
+template <class TT>
+__device__ int faceInsert(const TT& T, Shared& sh, int g, bool& fresh) {
+    fresh = false;
+    uint32_t h = hashOf((uint32_t)g) & T.fmask;
+    for (int p = 0; p <= T.fmask; ++p) {
+        int k = T.fkey[h];
+        if (k == g) return (int)h;
+        if (k == -1) {
+            if (atomicAdd(&sh.nFaces, 1) >= T.flimit) {
+                atomicAdd(&sh.nFaces, -1);
+                sh.overflow = 1;
+                return -1;
+            }
+            int old = atomicCAS(&T.fkey[h], -1, g);
+            if (old == -1) {
+                fresh = true;
+                return (int)h;
+            }
+            atomicAdd(&sh.nFaces, -1);
+            if (old == g) return (int)h;
+        }
+        h = (h + 1) & T.fmask;
+    }
+    sh.overflow = 1;
+    return -1;
+}
+
+template <class TT>
+__device__ int vertInsert(const TT& T, Shared& sh, int v) {
+    uint32_t h = hashOf((uint32_t)v) & T.vmask;
+    for (int p = 0; p <= T.vmask; ++p) {
+        int k = T.vkey[h];
+        if (k == v) return (int)h;
+        if (k == -1) {
+            if (atomicAdd(&sh.nVerts, 1) >= T.vlimit) {
+                atomicAdd(&sh.nVerts, -1);
+                sh.overflow = 1;
+                return -1;
+            }
+            int old = atomicCAS(&T.vkey[h], -1, v);
+            if (old == -1) return (int)h;
+            atomicAdd(&sh.nVerts, -1);
+            if (old == v) return (int)h;
+        }
+        h = (h + 1) & T.vmask;
+    }
+    sh.overflow = 1;
+    return -1;
+}
+
+__device__ inline void loadV(const DeviceMesh& M, int v, double* p) {
+    p[0] = M.vx[3 * v];
+    p[1] = M.vx[3 * v + 1];
+    p[2] = M.vx[3 * v + 2];
+}
+
+// start tangent (source-face frame) of the ray direction (rx, ry) of a window
+__device__ inline float2 startTangent(bool rooted, float2 rot, double rx, double ry) {
+    if (!rooted) return rot;
+    double l = sqrt(rx * rx + ry * ry);
+    if (l == 0.0) return make_float2(0.f, 0.f);
+    double ux = rx / l, uy = ry / l;
+    return make_float2((float)(rot.x * ux - rot.y * uy), (float)(rot.y * ux + rot.x * uy));
+}
+
+struct VCand {  // one pending vertex candidate of a lane, carried from pass 1 to pass 2
+    int slot;
+    double d;
+    float2 t;
+    bool improved;
+};
+
+__device__ inline VCand noCand() { return VCand{-1, 0.0, make_float2(0.f, 0.f), false}; }
+
+template <class TT>
+__device__ inline void offerVertex(const TT& T, VCand& c, int slot, double d, float2 t) {
+    if (slot < 0) return;
+    if (c.slot == slot && !(d < c.d)) return;
+    unsigned long long old = atomicMin(&T.vdist[slot], dbits(d));
+    bool imp = d < bitsd(old);
+    if (c.slot == slot) {
+        c.improved = c.improved || imp;
+    } else {
+        c.slot = slot;
+        c.improved = imp;
+    }
+    c.d = d;
+    c.t = t;
+}
+
+__device__ inline double keyOf(double b0, double b1, double sx, double sy, double sg) {
+    double dm;
+    if (sx < b0)
+        dm = sqrt((b0 - sx) * (b0 - sx) + sy * sy);
+    else if (sx > b1)
+        dm = sqrt((b1 - sx) * (b1 - sx) + sy * sy);
+    else
+        dm = -sy;
+    return sg + dm;
+}
+
+template <class TT>
+__device__ inline bool filtered(const TT& T, int vA, int vB, double L, double b0, double b1, double sx, double sy, double sg) {
+    double dA = bitsd(T.vdist[vA]), dB = bitsd(T.vdist[vB]);
+    double toI1 = sg + sqrt((b1 - sx) * (b1 - sx) + sy * sy);
+    double toI0 = sg + sqrt((b0 - sx) * (b0 - sx) + sy * sy);
+    if (toI1 > dA + b1 + kFilterTol * toI1) return true;
+    if (toI0 > dB + (L - b0) + kFilterTol * toI0) return true;
+    return false;
+}
+
+template <class TT>
+struct Ctx {
+    const GeoParams& P;
+    const TT& T;
+    Shared& sh;
+    int qBase;  // head at the start of the step: appends stay below qBase + qcap, so live records are never overwritten
+    double rMax;
+    int tb;     // first pair index of the source
+};
+
+// A new window on (face slot fs, local edge k): distance candidates for reached edge endpoints, filter, append.
+template <class TT>
+__device__ void addWindow(const Ctx<TT>& C, int fs, int k, double L, double b0, double b1, double sx, double sy, double sg, bool rooted,
+                          float2 rot, VCand& cA, VCand& cB) {
+    if (!(b1 > b0)) return;
+    if (keyOf(b0, b1, sx, sy, sg) > C.rMax) return;
+    const TT& T = C.T;
+    int vA = T.fcv[3 * fs + k], vB = T.fcv[3 * fs + (k + 1) % 3];
+    double tol = kEdgeTol * L;
+    if (b0 <= tol) offerVertex(T, cA, vA, sg + sqrt(sx * sx + sy * sy), startTangent(rooted, rot, -sx, -sy));
+    if (b1 >= L - tol) offerVertex(T, cB, vB, sg + sqrt((L - sx) * (L - sx) + sy * sy), startTangent(rooted, rot, L - sx, -sy));
+    if (filtered(T, vA, vB, L, b0, b1, sx, sy, sg)) return;
+    int pos = atomicAdd(&C.sh.tail, 1);
+    if (pos - C.qBase >= T.qcap) {
+        C.sh.overflow = 1;
+        return;
+    }
+    int q = pos & (T.qcap - 1);
+    T.qb0[q] = b0;
+    T.qb1[q] = b1;
+    T.qsx[q] = sx;
+    T.qsy[q] = sy;
+    T.qsg[q] = sg;
+    T.qfk[q] = fs * 8 + k * 2 + (rooted ? 1 : 0);
+    T.qt[q] = rot;
+}
+
+template <class TT>
+__device__ inline void pushVertexEvent(const Ctx<TT>& C, int vslot, double d) {
+    int pos = atomicAdd(&C.sh.tail, 1);
+    if (pos - C.qBase >= C.T.qcap) {
+        C.sh.overflow = 1;
+        return;
+    }
+    int q = pos & (C.T.qcap - 1);
+    C.T.qsg[q] = d;
+    C.T.qfk[q] = -vslot - 1;
+}
+
+// pseudo-source point Q (global coordinates) in the frame of the window entering face g through its edge k
+__device__ inline void pointInFrame(const DeviceMesh& M, int g, int k, const double* Q, double& sx, double& sy, double& L) {
+    double o[3], e[3];
+    loadV(M, M.fv[3 * g + k], o);
+    loadV(M, M.fv[3 * g + (k + 1) % 3], e);
+    L = M.fgeo[9 * g + 3 * k];
+    double ex[3] = {(e[0] - o[0]) / L, (e[1] - o[1]) / L, (e[2] - o[2]) / L};
+    double r[3] = {Q[0] - o[0], Q[1] - o[1], Q[2] - o[2]};
+    sx = r[0] * ex[0] + r[1] * ex[1] + r[2] * ex[2];
+    double c0 = r[1] * ex[2] - r[2] * ex[1], c1 = r[2] * ex[0] - r[0] * ex[2], c2 = r[0] * ex[1] - r[1] * ex[0];
+    sy = -sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    if (sy > -1e-14 * L) sy = -1e-14 * L;
+}
+
+// One window against the targets of the face it enters. Pass 1: atomic minimum. Pass 2: tangents of the winner.
+template <bool kPass2, class TT>
+__device__ void windowTargets(const Ctx<TT>& C, int fs, int g, int k, double L, double x2, double y2, double b0, double b1, double sx,
+                              double sy, double sg, bool rooted, float2 rot) {
+    const TT& T = C.T;
+    const GeoParams& P = C.P;
+    for (int t = T.fhead[fs]; t >= 0; t = T.tnext[t]) {
+        const double* b = P.pbary + 3 * P.tidx[C.tb + t];
+        double sb = b[0] + b[1] + b[2];
+        double bk1 = b[(k + 1) % 3] / sb, bk2 = b[(k + 2) % 3] / sb;
+        double qx = bk1 * L + bk2 * x2, qy = bk2 * y2;
+        double rx = qx - sx, ry = qy - sy;
+        double ax = b0 - sx, ay = -sy, bx = b1 - sx, by = -sy;
+        double rl = sqrt(rx * rx + ry * ry);
+        double c1 = bx * ry - by * rx;  // cross(interval end ray, target ray)
+        double c2 = rx * ay - ry * ax;  // cross(target ray, interval start ray)
+        double tol = 1e-12 * rl;
+        if (c1 < -tol * sqrt(bx * bx + by * by) || c2 < -tol * sqrt(ax * ax + ay * ay)) continue;
+        double d = sg + rl;
+        if (!kPass2) {
+            atomicMin(&T.tbest[t], dbits(d));
+        } else if (T.tbest[t] == dbits(d)) {
+            T.tst[t] = startTangent(rooted, rot, rx, ry);
+            const DeviceMesh& M = P.mesh;
+            double o[3], e[3];
+            loadV(M, M.fv[3 * g + k], o);
+            loadV(M, M.fv[3 * g + (k + 1) % 3], e);
+            double ex[3] = {(e[0] - o[0]) / L, (e[1] - o[1]) / L, (e[2] - o[2]) / L};
+            const double* n = M.fnrm + 3 * g;
+            double ey[3] = {n[1] * ex[2] - n[2] * ex[1], n[2] * ex[0] - n[0] * ex[2], n[0] * ex[1] - n[1] * ex[0]};
+            T.tet[t] = make_float4((float)((rx * ex[0] + ry * ey[0]) / rl), (float)((rx * ex[1] + ry * ey[1]) / rl),
+                                   (float)((rx * ex[2] + ry * ey[2]) / rl), 0.f);
+        }
+    }
+}
+
+// Window propagation across the face it enters (pass 1): candidate for the opposite corner, up to two children.
+template <class TT>
+__device__ void propagate(const Ctx<TT>& C, int fk, double b0, double b1, double sx, double sy, double sg, float2 rot, VCand& cP0,
+                          VCand& cP1, VCand& cP2) {
+    const TT& T = C.T;
+    const DeviceMesh& M = C.P.mesh;
+    int fs = fk >> 3, k = (fk >> 1) & 3;
+    bool rooted = fk & 1;
+    int g = T.fkey[fs];
+    const double* geo = M.fgeo + 9 * g + 3 * k;
+    double L = geo[0], x2 = geo[1], y2 = geo[2];
+    int vC = T.fcv[3 * fs + (k + 2) % 3];
+    windowTargets<false, TT>(C, fs, g, k, L, x2, y2, b0, b1, sx, sy, sg, rooted, rot);
+    double xs = sx + (x2 - sx) * (-sy) / (y2 - sy);  // where the ray through the opposite corner crosses the entry edge
+    double tolE = kEdgeTol * L;
+    if (xs >= b0 - tolE && xs <= b1 + tolE)
+        offerVertex(T, cP2, vC, sg + sqrt((x2 - sx) * (x2 - sx) + (y2 - sy) * (y2 - sy)), startTangent(rooted, rot, x2 - sx, y2 - sy));
+    if (xs > b0) {  // left child: edge corner k -> corner k+2, entered from the neighbour across local edge k+2
+        int nb = T.fnb[3 * fs + (k + 2) % 3];
+        if (nb >= 0) {
+            double lL = sqrt(x2 * x2 + y2 * y2);
+            double ux = x2 / lL, uy = y2 / lL;
+            double spx = sx * ux + sy * uy, spy = ux * sy - uy * sx;
+            auto hit = [&](double bx) {
+                double dx = bx - sx, dy = -sy;
+                double dpx = dx * ux + dy * uy, dpy = ux * dy - uy * dx;
+                if (dpy <= 0) return dpx < 0 ? 0.0 : lL;
+                return fmin(lL, fmax(0.0, spx + dpx * (-spy) / dpy));
+            };
+            double t0 = (b0 <= 0) ? 0.0 : hit(b0);
+            double t1 = (xs <= b1) ? lL : hit(b1);
+            float2 r2 = rooted ? make_float2((float)(rot.x * ux - rot.y * uy), (float)(rot.y * ux + rot.x * uy)) : rot;
+            addWindow(C, nb >> 2, nb & 3, lL, t0, t1, spx, fmin(spy, -1e-14 * lL), sg, rooted, r2, cP0, cP2);
+        }
+    }
+    if (xs < b1) {  // right child: edge corner k+2 -> corner k+1, entered from the neighbour across local edge k+1
+        int nb = T.fnb[3 * fs + (k + 1) % 3];
+        if (nb >= 0) {
+            double ex = L - x2, ey = -y2;
+            double lR = sqrt(ex * ex + ey * ey);
+            double ux = ex / lR, uy = ey / lR;
+            double rx0 = sx - x2, ry0 = sy - y2;
+            double spx = rx0 * ux + ry0 * uy, spy = ux * ry0 - uy * rx0;
+            auto hit = [&](double bx) {
+                double dx = bx - sx, dy = -sy;
+                double dpx = dx * ux + dy * uy, dpy = ux * dy - uy * dx;
+                if (dpy <= 0) return dpx < 0 ? 0.0 : lR;
+                return fmin(lR, fmax(0.0, spx + dpx * (-spy) / dpy));
+            };
+            double t0 = (xs >= b0) ? 0.0 : hit(b0);
+            double t1 = (b1 >= L) ? lR : hit(b1);
+            float2 r2 = rooted ? make_float2((float)(rot.x * ux - rot.y * uy), (float)(rot.y * ux + rot.x * uy)) : rot;
+            addWindow(C, nb >> 2, nb & 3, lR, t0, t1, spx, fmin(spy, -1e-14 * lR), sg, rooted, r2, cP2, cP1);
+        }
+    }
+}
+
+// Expansion of a pseudo-source vertex: a window on the far edge of every incident patch face. Pass 1 appends the
+// windows and records in `improved` (two bits per incident face) which endpoint distances it lowered; pass 2 gives
+// those endpoints the vertex's tangent and queues events for the ones that are pseudo-sources themselves.
+template <class TT>
+__device__ void expandVertex(const Ctx<TT>& C, int vslot, double d, bool pass2, unsigned long long& improved) {
+    const TT& T = C.T;
+    const DeviceMesh& M = C.P.mesh;
+    int v = T.vkey[vslot];
+    double Q[3];
+    loadV(M, v, Q);
+    float2 tv = T.vtan[vslot];
+    const int q0 = M.vfoff[v], q1 = min(M.vfoff[v + 1], q0 + 32);
+    for (int q = q0; q < q1; ++q) {
+        int g = M.vflist[q] >> 2, j = M.vflist[q] & 3;
+        int gs = faceLookup(T, g);
+        if (gs < 0) continue;
+        int nb = T.fnb[3 * gs + (j + 1) % 3];
+        if (nb < 0) continue;
+        int hs = nb >> 2, kh = nb & 3;
+        double sx, sy, L;
+        pointInFrame(M, T.fkey[hs], kh, Q, sx, sy, L);
+        const int bit = 2 * (q - q0);
+        if (!pass2) {
+            VCand a = noCand(), b = noCand();
+            addWindow(C, hs, kh, L, 0.0, L, sx, sy, d, false, tv, a, b);
+            if (a.improved) improved |= 1ull << bit;
+            if (b.improved) improved |= 2ull << bit;
+        } else {
+            int ends[2] = {T.fcv[3 * hs + kh], T.fcv[3 * hs + (kh + 1) % 3]};
+            double dd[2] = {d + sqrt(sx * sx + sy * sy), d + sqrt((L - sx) * (L - sx) + sy * sy)};
+            for (int e = 0; e < 2; ++e) {
+                if (!((improved >> (bit + e)) & 1)) continue;
+                int u = ends[e];
+                if (T.vdist[u] != dbits(dd[e])) continue;
+                T.vtan[u] = tv;
+                if (T.vflg[u] && dd[e] <= C.rMax) pushVertexEvent(C, u, dd[e]);
+            }
+        }
+    }
+}
+
+template <class TT>
+__device__ inline void finishVertexCand(const Ctx<TT>& C, const VCand& c) {
+    if (c.slot < 0 || !c.improved) return;
+    if (C.T.vdist[c.slot] != dbits(c.d)) return;
+    C.T.vtan[c.slot] = c.t;
+    if (C.T.vflg[c.slot] && c.d <= C.rMax) pushVertexEvent(C, c.slot, c.d);
+}
+
+__device__ inline double wsum(double x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+// Builds the patch and the per-slot tables. Returns false (with sh.overflow set) when a capacity is exceeded.
+template <class TT>
+__device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int src, int sf, int tb, int nT) {
+    const int lane = threadIdx.x;
+    const DeviceMesh& M = P.mesh;
+    int* cur = (int*)T.qb0;  // frontier lists borrow the FIFO storage (two ints per double)
+    int* nxt = (int*)T.qb1;
+    const int frontCap = 2 * T.qcap;
+    if (P.patchRadius == nullptr) {
+        for (int g = lane; g < M.nF; g += 64) {
+            bool fresh;
+            faceInsert(T, sh, g, fresh);
+        }
+    } else {
+        const double r = P.patchRadius[src];
+        const double r2 = r * r;
+        int nbr[3];
+        for (int k = 0; k < 3; ++k) nbr[k] = M.fadj[3 * sf + k] < 0 ? -2 : (M.fadj[3 * sf + k] >> 2);
+        int anyGoal = 0, beyond = 0;
+        for (int t = lane; t < nT; t += 64) {
+            int f = P.pface[P.tidx[tb + t]];
+            if (f != sf) anyGoal = 1;
+            if (f != sf && f != nbr[0] && f != nbr[1] && f != nbr[2]) beyond = 1;
+        }
+        if (anyGoal) atomicOr(&sh.goals, 1);
+        if (beyond) atomicOr(&sh.goals, 2);
+        __syncthreads();
+        if (lane == 0) {
+            bool fresh;
+            faceInsert(T, sh, sf, fresh);
+            if (sh.goals & 1)
+                for (int k = 0; k < 3; ++k)
+                    if (nbr[k] >= 0) {
+                        faceInsert(T, sh, nbr[k], fresh);
+                        if (fresh) cur[sh.nCur++] = nbr[k];
+                    }
+        }
+        __syncthreads();
+        if (sh.goals & 2) {
+            // breadth-first: the set reached equals the reference's depth-first set (order-independent closure)
+            while (true) {
+                const int nc = sh.nCur;
+                if (nc == 0 || sh.overflow) break;
+                for (int i = lane; i < nc; i += 64) {
+                    int f = cur[i];
+                    for (int k = 0; k < 3; ++k) {
+                        int a = M.fadj[3 * f + k];
+                        if (a < 0) continue;
+                        int g = a >> 2;
+                        if (faceLookup(T, g) >= 0) continue;
+                        bool outside = true;
+                        for (int j = 0; j < 3; ++j) {
+                            double p[3];
+                            loadV(M, M.fv[3 * g + j], p);
+                            double dx = p[0] - sh.S[0], dy = p[1] - sh.S[1], dz = p[2] - sh.S[2];
+                            if (!(dx * dx + dy * dy + dz * dz > r2)) outside = false;
+                        }
+                        if (outside) continue;
+                        bool fresh;
+                        int s = faceInsert(T, sh, g, fresh);
+                        if (s >= 0 && fresh) {
+                            int pos = atomicAdd(&sh.nNext, 1);
+                            if (pos < frontCap)
+                                nxt[pos] = g;
+                            else
+                                sh.overflow = 1;
+                        }
+                    }
+                }
+                __syncthreads();
+                const int nn = min(sh.nNext, frontCap);
+                for (int i = lane; i < nn; i += 64) cur[i] = nxt[i];
+                __syncthreads();
+                if (lane == 0) {
+                    sh.nCur = nn;
+                    sh.nNext = 0;
+                }
+                __syncthreads();
+            }
+        }
+        // target faces not reached above (possibly disconnected from the source)
+        for (int t = lane; t < nT; t += 64) {
+            bool fresh;
+            faceInsert(T, sh, P.pface[P.tidx[tb + t]], fresh);
+        }
+    }
+    __syncthreads();
+    if (sh.overflow) return false;
+    for (int s = lane; s <= T.fmask; s += 64) {
+        int g = T.fkey[s];
+        if (g < 0) continue;
+        for (int k = 0; k < 3; ++k) {
+            T.fcv[3 * s + k] = vertInsert(T, sh, M.fv[3 * g + k]);
+            int a = M.fadj[3 * g + k];
+            int ns = a < 0 ? -1 : faceLookup(T, a >> 2);
+            T.fnb[3 * s + k] = ns < 0 ? -1 : ns * 4 + (a & 3);
+        }
+    }
+    __syncthreads();
+    if (sh.overflow) return false;
+    for (int s = lane; s <= T.vmask; s += 64) {
+        int v = T.vkey[s];
+        if (v < 0) continue;
+        int fl = (M.vflag[v] & 3) ? 1 : 0;
+        if (!fl)
+            for (int q = M.vfoff[v]; q < M.vfoff[v + 1]; ++q)
+                if (faceLookup(T, M.vflist[q] >> 2) < 0) {
+                    fl = 1;
+                    break;
+                }
+        T.vflg[s] = fl;
+        T.vdist[s] = dbits(kInf);
+        T.vtan[s] = make_float2(0.f, 0.f);
+    }
+    // per-face target lists: slots looked up in parallel (parked in tbest), linked by one lane
+    for (int t = lane; t < nT; t += 64) T.tbest[t] = (unsigned long long)faceLookup(T, P.pface[P.tidx[tb + t]]);
+    __syncthreads();
+    if (lane == 0)
+        for (int t = nT - 1; t >= 0; --t) {
+            int s = (int)T.tbest[t];
+            T.tnext[t] = T.fhead[s];
+            T.fhead[s] = t;
+        }
+    __syncthreads();
+    for (int t = lane; t < nT; t += 64) T.tbest[t] = dbits(kInf);
+    __syncthreads();
+    return true;
+}
+
+template <class TT>
+__device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf, int tb) {
+    const int lane = threadIdx.x;
+    const DeviceMesh& M = P.mesh;
+    Ctx<TT> C{P, T, sh, 0, P.rMax, tb};
+    const int ss = faceLookup(T, sf);
+    // corners of the source face are visible directly
+    if (lane < 3) {
+        double p[3];
+        loadV(M, M.fv[3 * sf + lane], p);
+        double r[3] = {p[0] - sh.S[0], p[1] - sh.S[1], p[2] - sh.S[2]};
+        double l = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        int vs = T.fcv[3 * ss + lane];
+        T.vdist[vs] = dbits(l);  // three distinct corners, one lane each
+        T.vtan[vs] = l > 0 ? make_float2((float)((r[0] * sh.t1[0] + r[1] * sh.t1[1] + r[2] * sh.t1[2]) / l),
+                                         (float)((r[0] * sh.t2[0] + r[1] * sh.t2[1] + r[2] * sh.t2[2]) / l))
+                           : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    if (lane < 3) {
+        int vs = T.fcv[3 * ss + lane];
+        double d = bitsd(T.vdist[vs]);
+        if (T.vflg[vs] && d <= C.rMax) pushVertexEvent(C, vs, d);
+        int nb = T.fnb[3 * ss + lane];
+        if (nb >= 0) {
+            int hs = nb >> 2, kh = nb & 3, h = T.fkey[hs];
+            double sx, sy, L;
+            pointInFrame(M, h, kh, sh.S, sx, sy, L);
+            double o[3], e[3];
+            loadV(M, M.fv[3 * h + kh], o);
+            loadV(M, M.fv[3 * h + (kh + 1) % 3], e);
+            double ex[3] = {(e[0] - o[0]) / L, (e[1] - o[1]) / L, (e[2] - o[2]) / L};
+            float2 rot = make_float2((float)(ex[0] * sh.t1[0] + ex[1] * sh.t1[1] + ex[2] * sh.t1[2]),
+                                     (float)(ex[0] * sh.t2[0] + ex[1] * sh.t2[1] + ex[2] * sh.t2[2]));
+            VCand a = noCand(), b = noCand();
+            addWindow(C, hs, kh, L, 0.0, L, sx, sy, 0.0, true, rot, a, b);
+        }
+    }
+    __syncthreads();
+    for (int step = 0;; ++step) {
+        const int h = sh.head, t = sh.tail;
+        if (h >= t || sh.overflow) break;
+        if (step >= kMaxSteps) {  // every source ends: an unfinished one is reported like a capacity overflow
+            if (lane == 0) sh.overflow = 1;
+            break;
+        }
+        const int nb = min(64, t - h);
+        C.qBase = h;
+        const bool mine = lane < nb;
+        const int q = (h + lane) & (T.qcap - 1);
+        int fk = 0;
+        double b0 = 0, b1 = 0, sx = 0, sy = 0, sg = 0;
+        float2 rot = make_float2(0.f, 0.f);
+        if (mine) {
+            fk = T.qfk[q];
+            sg = T.qsg[q];
+            if (fk >= 0) {
+                b0 = T.qb0[q];
+                b1 = T.qb1[q];
+                sx = T.qsx[q];
+                sy = T.qsy[q];
+                rot = T.qt[q];
+            }
+        }
+        bool live = false;
+        if (mine) {
+            if (fk >= 0) {
+                int fs = fk >> 3, k = (fk >> 1) & 3;
+                double L = M.fgeo[9 * T.fkey[fs] + 3 * k];
+                live = keyOf(b0, b1, sx, sy, sg) <= C.rMax &&
+                       !filtered(T, T.fcv[3 * fs + k], T.fcv[3 * fs + (k + 1) % 3], L, b0, b1, sx, sy, sg);
+            } else {
+                live = T.vdist[-fk - 1] == dbits(sg) && sg <= C.rMax;
+            }
+        }
+        __syncthreads();
+        VCand cP0 = noCand(), cP1 = noCand(), cP2 = noCand();
+        unsigned long long improved = 0;
+        if (live) {
+            if (fk >= 0)
+                propagate(C, fk, b0, b1, sx, sy, sg, rot, cP0, cP1, cP2);
+            else
+                expandVertex(C, -fk - 1, sg, false, improved);
+        }
+        __syncthreads();
+        if (live) {
+            if (fk >= 0) {
+                int fs = fk >> 3;
+                if (T.fhead[fs] >= 0) {
+                    int k = (fk >> 1) & 3;
+                    int g = T.fkey[fs];
+                    const double* geo = M.fgeo + 9 * g + 3 * k;
+                    windowTargets<true, TT>(C, fs, g, k, geo[0], geo[1], geo[2], b0, b1, sx, sy, sg, fk & 1, rot);
+                }
+                finishVertexCand(C, cP0);
+                finishVertexCand(C, cP1);
+                finishVertexCand(C, cP2);
+            } else {
+                expandVertex(C, -fk - 1, sg, true, improved);
+            }
+        }
+        __syncthreads();
+        if (lane == 0) sh.head = h + nb;
+        __syncthreads();
+    }
+}
+
+template <class TT>
+__device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src) {
+    const int lane = threadIdx.x;
+    const DeviceMesh& M = P.mesh;
+    const int pi = P.srcIdx ? P.srcIdx[src] : src;
+    const int sf = P.pface[pi];
+    const int tb = P.toff[src], nT = P.toff[src + 1] - tb;
+    for (int i = lane; i <= T.fmask; i += 64) {
+        T.fkey[i] = -1;
+        T.fhead[i] = -1;
+    }
+    for (int i = lane; i <= T.vmask; i += 64) T.vkey[i] = -1;
+    for (int i = lane; i < min(nT, T.tcap); i += 64) T.tbest[i] = dbits(kInf);
+    if (lane == 0) {
+        sh.head = sh.tail = sh.nFaces = sh.nVerts = 0;
+        sh.overflow = nT > T.tcap ? 1 : 0;
+        sh.nCur = sh.nNext = sh.goals = 0;
+        const double* bb = P.pbary + 3 * pi;
+        double s = bb[0] + bb[1] + bb[2];
+        double p0[3], p1[3], p2[3];
+        loadV(M, M.fv[3 * sf], p0);
+        loadV(M, M.fv[3 * sf + 1], p1);
+        loadV(M, M.fv[3 * sf + 2], p2);
+        for (int a = 0; a < 3; ++a) sh.S[a] = (p0[a] * bb[0] + p1[a] * bb[1] + p2[a] * bb[2]) / s;
+        double e[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+        double l = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+        for (int a = 0; a < 3; ++a) sh.t1[a] = e[a] / l;
+        const double* n = M.fnrm + 3 * sf;
+        sh.t2[0] = n[1] * sh.t1[2] - n[2] * sh.t1[1];
+        sh.t2[1] = n[2] * sh.t1[0] - n[0] * sh.t1[2];
+        sh.t2[2] = n[0] * sh.t1[1] - n[1] * sh.t1[0];
+    }
+    __syncthreads();
+    bool ok = !sh.overflow;
+    __syncthreads();
+    if (ok) ok = buildPatch(P, T, sh, src, sf, tb, nT);
+    if (ok) propagateAll(P, T, sh, sf, tb);
+    __syncthreads();
+    // ---- finalize: paths through the target face's corners, direct visibility, outputs, force
+    const bool ovf = sh.overflow != 0;
+    double F0 = 0, F1 = 0, F2 = 0, E = 0;
+    for (int t = lane; t < nT; t += 64) {
+        const int pj = P.tidx[tb + t];
+        const int out = tb + t;
+        if (ovf) {
+            if (P.pairFlags) P.pairFlags[out] = PAIR_OVERFLOW;
+            continue;
+        }
+        const int f = P.pface[pj];
+        const double* b = P.pbary + 3 * pj;
+        double sb = b[0] + b[1] + b[2];
+        double cc[3][3];
+        for (int k = 0; k < 3; ++k) loadV(M, M.fv[3 * f + k], cc[k]);
+        double Q[3];
+        for (int a = 0; a < 3; ++a) Q[a] = (cc[0][a] * b[0] + cc[1][a] * b[1] + cc[2][a] * b[2]) / sb;
+        double best = bitsd(T.tbest[t]);
+        float2 st = T.tst[t];
+        float4 et4 = T.tet[t];
+        double et[3] = {et4.x, et4.y, et4.z};
+        if (f == sf) {
+            double r[3] = {Q[0] - sh.S[0], Q[1] - sh.S[1], Q[2] - sh.S[2]};
+            double l = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+            if (l < best) {
+                best = l;
+                if (l > 0) {
+                    st = make_float2((float)((r[0] * sh.t1[0] + r[1] * sh.t1[1] + r[2] * sh.t1[2]) / l),
+                                     (float)((r[0] * sh.t2[0] + r[1] * sh.t2[1] + r[2] * sh.t2[2]) / l));
+                    for (int a = 0; a < 3; ++a) et[a] = r[a] / l;
+                } else {
+                    st = make_float2(0.f, 0.f);
+                    et[0] = et[1] = et[2] = 0;
+                }
+            }
+        }
+        const int fs = faceLookup(T, f);
+        for (int k = 0; k < 3 && fs >= 0; ++k) {
+            int vs = T.fcv[3 * fs + k];
+            double dv = bitsd(T.vdist[vs]);
+            if (!(dv < kInf)) continue;
+            double r[3] = {Q[0] - cc[k][0], Q[1] - cc[k][1], Q[2] - cc[k][2]};
+            double l = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+            if (dv + l < best) {
+                best = dv + l;
+                st = T.vtan[vs];
+                for (int a = 0; a < 3; ++a) et[a] = l > 0 ? r[a] / l : 0.0;
+            }
+        }
+        uint32_t flags = 0;
+        double d = best;
+        double s3[3] = {st.x * sh.t1[0] + st.y * sh.t2[0], st.x * sh.t1[1] + st.y * sh.t2[1], st.x * sh.t1[2] + st.y * sh.t2[2]};
+        double nn = sqrt(s3[0] * s3[0] + s3[1] * s3[1] + s3[2] * s3[2]);
+        if (nn > 0)
+            for (int a = 0; a < 3; ++a) s3[a] /= nn;
+        if (!(best < kInf)) {
+            flags |= PAIR_UNREACHED;
+            s3[0] = s3[1] = s3[2] = 0;
+            et[0] = et[1] = et[2] = 0;
+            if (P.useSentinel) {
+                flags |= PAIR_SENTINEL;
+                d = P.sentinelDist;
+                s3[2] = 1;
+                et[2] = 1;
+            }
+        } else if (best > P.rMax) {
+            flags |= PAIR_BEYOND;
+        }
+        if (P.pairDist) P.pairDist[out] = d;
+        if (P.pairStart) {
+            P.pairStart[3 * out] = (float)s3[0];
+            P.pairStart[3 * out + 1] = (float)s3[1];
+            P.pairStart[3 * out + 2] = (float)s3[2];
+        }
+        if (P.pairEnd) {
+            P.pairEnd[3 * out] = (float)et[0];
+            P.pairEnd[3 * out + 1] = (float)et[1];
+            P.pairEnd[3 * out + 2] = (float)et[2];
+        }
+        if (P.pairFlags) P.pairFlags[out] = flags;
+        if (P.pot.kind != POT_NONE && (!(flags & PAIR_UNREACHED) || (flags & PAIR_SENTINEL))) {
+            double e;
+            double sc = potentialScale(P.pot, d, &e);
+            F0 += sc * s3[0];
+            F1 += sc * s3[1];
+            F2 += sc * s3[2];
+            E += e;
+        }
+    }
+    F0 = wsum(F0);
+    F1 = wsum(F1);
+    F2 = wsum(F2);
+    E = wsum(E);
+    if (lane == 0) {
+        if (P.forces) {
+            P.forces[3 * src] = F0;
+            P.forces[3 * src + 1] = F1;
+            P.forces[3 * src + 2] = F2;
+        }
+        if (P.energy) P.energy[src] = E;
+        if (P.status) P.status[src] = ovf ? 1 : 0;
+    }
+    __syncthreads();
+}
+
+}  // namespace
+
+// LDS variant: tables in __shared__ memory, one source per workgroup
+template <int FH, int VH, int QC, int TC>
+__global__ __launch_bounds__(64) void geodesicForceLds(GeoParams P) {
+    __shared__ int32_t fkey[FH];
+    __shared__ int16_t fhead[FH], fcv[3 * FH], fnb[3 * FH];
+    __shared__ int32_t vkey[VH], vflg[VH];
+    __shared__ unsigned long long vdist[VH];
+    __shared__ float2 vtan[VH];
+    __shared__ double qb0[QC], qb1[QC], qsx[QC], qsy[QC], qsg[QC];
+    __shared__ int32_t qfk[QC];
+    __shared__ float2 qt[QC];
+    __shared__ int16_t tnext[TC];
+    __shared__ unsigned long long tbest[TC];
+    __shared__ float2 tst[TC];
+    __shared__ float4 tet[TC];
+    __shared__ Shared sh;
+    static_assert(2 * QC >= FH / 2, "BFS frontiers live in the FIFO storage");
+    static_assert(4 * FH <= 32767 && VH <= 32767 && TC <= 32767, "slots must fit int16");
+    Tables<int16_t> T{fkey, fcv, fnb, fhead, vkey, vdist, vtan, vflg, qb0, qb1, qsx, qsy, qsg, qfk, qt, tnext, tbest, tst, tet,
+             FH - 1, VH - 1, QC, TC, FH / 2, (VH * 3) / 4};
+    const int src = blockIdx.x;
+    if (src >= P.nSrc) return;
+    solveSource(P, T, sh, src);
+}
+
+// Scratch variant: the same solver with its tables in a per-workgroup slice of an HBM buffer, persistent over a
+// list of sources (sources the LDS variant could not hold, or whole-mesh solves).
+__global__ __launch_bounds__(64) void geodesicForceScratch(GeoParams P, ScratchLayout L, const int32_t* list, int nList) {
+    __shared__ Shared sh;
+    char* base = (char*)L.base + (size_t)blockIdx.x * L.bytesPerBlock;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* p = base + off;
+        off += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    Tables<int32_t> T;
+    T.fkey = (int32_t*)take(4 * (size_t)L.fcap);
+    T.fhead = (int32_t*)take(4 * (size_t)L.fcap);
+    T.fcv = (int32_t*)take(12 * (size_t)L.fcap);
+    T.fnb = (int32_t*)take(12 * (size_t)L.fcap);
+    T.vkey = (int32_t*)take(4 * (size_t)L.vcap);
+    T.vflg = (int32_t*)take(4 * (size_t)L.vcap);
+    T.vdist = (unsigned long long*)take(8 * (size_t)L.vcap);
+    T.vtan = (float2*)take(8 * (size_t)L.vcap);
+    T.qb0 = (double*)take(8 * (size_t)L.qcap);
+    T.qb1 = (double*)take(8 * (size_t)L.qcap);
+    T.qsx = (double*)take(8 * (size_t)L.qcap);
+    T.qsy = (double*)take(8 * (size_t)L.qcap);
+    T.qsg = (double*)take(8 * (size_t)L.qcap);
+    T.qfk = (int32_t*)take(4 * (size_t)L.qcap);
+    T.qt = (float2*)take(8 * (size_t)L.qcap);
+    T.tnext = (int32_t*)take(4 * (size_t)L.tcap);
+    T.tbest = (unsigned long long*)take(8 * (size_t)L.tcap);
+    T.tst = (float2*)take(8 * (size_t)L.tcap);
+    T.tet = (float4*)take(16 * (size_t)L.tcap);
+    T.fmask = L.fcap - 1;
+    T.vmask = L.vcap - 1;
+    T.qcap = L.qcap;
+    T.tcap = L.tcap;
+    T.flimit = L.fcap / 2;
+    T.vlimit = (L.vcap * 3) / 4;
+    for (int i = blockIdx.x; i < nList; i += gridDim.x) solveSource(P, T, sh, list ? list[i] : i);
+}
+
+size_t scratchBytesPerBlock(int fcap, int vcap, int qcap, int tcap) {
+    auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return 2 * r(4 * (size_t)fcap) + 2 * r(12 * (size_t)fcap) + 2 * r(4 * (size_t)vcap) + 2 * r(8 * (size_t)vcap) + 5 * r(8 * (size_t)qcap) +
+           r(4 * (size_t)qcap) + r(8 * (size_t)qcap) + r(4 * (size_t)tcap) + 2 * r(8 * (size_t)tcap) + r(16 * (size_t)tcap);
+}
+
+hipError_t launchGeodesicLds(const GeoParams& P, hipStream_t stream) {
+    if (P.nSrc <= 0) return hipSuccess;
+    hipLaunchKernelGGL((geodesicForceLds<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap>), dim3(P.nSrc), dim3(64), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launchGeodesicScratch(const GeoParams& P, const ScratchLayout& L, const int32_t* list, int nList, int blocks,
+                                 hipStream_t stream) {
+    if (nList <= 0) return hipSuccess;
+    hipLaunchKernelGGL(geodesicForceScratch, dim3(blocks), dim3(64), 0, stream, P, L, list, nList);
+    return hipGetLastError();
+}
+
+}  // namespace csg

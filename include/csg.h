@@ -1,0 +1,104 @@
+/* libcsg.so — C-ABI of the MI355X (gfx950) geodesic force path.
+ *
+ * Drop-in boundary for the mesh-space calls of curvedSpaceSim: every entry point below replaces a reference interface
+ * (cited per function, paths relative to the reference tree) with a batched, device-backed equivalent. No C++ types
+ * cross this boundary; arrays are plain host pointers (the call copies them to and from HBM) unless a function says
+ * the data stays resident on the device. No function throws; each returns a csg_status and csg_last_error() describes
+ * the last failure of a context. Calls on one context must not overlap (one context per host thread).
+ *
+ * Conventions (those of the reference):
+ *   - a mesh position is a face index plus three barycentric weights in the order of the face's corners
+ *     (inc/pointDataType.h:10-28; triangles are given counter-clockwise seen from outside);
+ *   - tangents are unit vectors; the start tangent points from the source along the path, the end tangent points
+ *     away from the source at the target (src/utility/meshUtilities.cpp:373-379);
+ *   - with submeshing on, a target that the patch does not connect to the source gets distance 2*maximumDistance and
+ *     tangents (0,0,1) (src/models/triangulatedMeshSpace.cpp:197-203).
+ */
+#ifndef CSG_H
+#define CSG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct csg_ctx csg_ctx;
+
+typedef enum csg_status {
+    CSG_OK = 0,
+    CSG_ERR_ARGUMENT = -1,   /* bad argument (null pointer, size, index out of range) */
+    CSG_ERR_DEVICE = -2,     /* HIP runtime error, or no gfx950 device */
+    CSG_ERR_MESH = -3,       /* not a consistently oriented 2-manifold triangle mesh (triangulatedMeshSpace.cpp:51-63) */
+    CSG_ERR_STATE = -4,      /* call order: no mesh, no particles, ... */
+    CSG_ERR_WALK = -5,       /* a displacement walk failed as the reference throws (triangulatedMeshSpace.cpp:522-548,
+                                meshUtilities.cpp:184-195); the per-particle status array names the case */
+    CSG_ERR_CAPACITY = -6    /* an input exceeds a capacity of this build (documented per function) */
+} csg_status;
+
+/* pair potentials of the force computers (src/forces/): kind 0 harmonicRepulsion (p0 = k, p1 = sigma,
+ * harmonicRepulsion.cpp:3-34), 1 gaussianRepulsion (p0 = alpha, p1 = sigma, gaussianRepulsion.cpp:3-12),
+ * 2 truncated Gaussian (p0 = epsilon, p1 = s, p2 = cutoff; U = epsilon exp(-d^2 / 2 s^2) for d < cutoff, no reference
+ * counterpart), 3 none. */
+typedef struct csg_potential {
+    int32_t kind;
+    double p0, p1, p2;
+} csg_potential;
+
+/* per-pair flags */
+#define CSG_PAIR_UNREACHED 1u /* no path inside the patch (or none shorter than rMax) */
+#define CSG_PAIR_BEYOND 4u    /* a path exists but is longer than rMax */
+#define CSG_PAIR_SENTINEL 16u /* unreachable pair reported with the reference sentinel */
+
+/* --- context ------------------------------------------------------------------------------------------------------ */
+csg_status csg_ctx_create(csg_ctx** out, int device);
+void csg_ctx_destroy(csg_ctx* ctx);
+const char* csg_last_error(const csg_ctx* ctx);
+/* name of the device architecture the context runs on (e.g. "gfx950") */
+const char* csg_device_name(const csg_ctx* ctx);
+
+/* --- mesh: triangulatedMeshSpace::loadMeshFromFile (src/models/triangulatedMeshSpace.cpp:42-72) ----------------- */
+csg_status csg_mesh_upload(csg_ctx* ctx, int32_t nV, const double* xyz, int32_t nF, const int32_t* tri);
+/* getArea (triangulatedMeshSpace.cpp:153) and minVertexPosition / maxVertexPosition (:6-30, the box is seeded at the
+ * origin as the reference does) */
+csg_status csg_mesh_info(const csg_ctx* ctx, int32_t* nV, int32_t* nF, double* area, double* lo3, double* hi3);
+
+/* --- triangulatedMeshSpace::useSubmeshingRoutines (src/models/triangulatedMeshSpace.h:60-66) --------------------- */
+csg_status csg_set_submeshing(csg_ctx* ctx, int32_t enabled, double maximumDistance);
+
+/* --- triangulatedMeshSpace::distance (src/models/triangulatedMeshSpace.cpp:155-238), batched over sources ---------
+ * Source s has targets tgtOffsets[s] .. tgtOffsets[s+1]-1. distanceThreshold (nullable: 1e20 each) is the per-source
+ * argument of the reference call; with submeshing the patch radius is min(maximumDistance, distanceThreshold[s]).
+ * Outputs are per target: dist (nullable), startTan and endTan (3 each, nullable), flags (nullable).
+ * Capacity: with submeshing, at most 2^20 targets per source; whole-mesh mode needs 2*nF hash slots per workgroup. */
+csg_status csg_distance_batch(csg_ctx* ctx, int32_t nSrc, const int32_t* srcFace, const double* srcBary, const int32_t* tgtOffsets,
+                              const int32_t* tgtFace, const double* tgtBary, const double* distanceThreshold, double* dist,
+                              double* startTan, double* endTan, uint32_t* flags);
+
+/* --- fused simpleModel::findNeighbors + force::computeForces (src/models/simpleModel.cpp:68-112,
+ * src/forces/baseForce.cpp:12-44) over explicit neighbour lists of particles given by position -------------------
+ * Particles: nP positions. Source s is particle srcIdx[s] (nullable: s) with candidate particles
+ * tgtIdx[tgtOffsets[s] .. tgtOffsets[s+1]-1]; patchRadius (nullable: whole mesh) per source; windows whose distance
+ * lower bound exceeds rMax are not propagated (pass a huge value for none). Unreached pairs get the reference
+ * sentinel when useSentinel != 0. forces: 3 per source, sum over its pairs of pairForce(startTangent, distance);
+ * energy: per source sum of pair energies. Every output is nullable. */
+csg_status csg_pair_forces(csg_ctx* ctx, int32_t nP, const int32_t* face, const double* bary, int32_t nSrc, const int32_t* srcIdx,
+                           const int32_t* tgtOffsets, const int32_t* tgtIdx, const double* patchRadius, double rMax,
+                           const csg_potential* pot, int32_t useSentinel, double sentinelDistance, double* dist, float* startTan,
+                           float* endTan, uint32_t* flags, double* forces, double* energy);
+
+/* --- triangulatedMeshSpace::transportParticleAndVectors (src/models/triangulatedMeshSpace.cpp:448-564), batched -----
+ * n particles; face, bary (3 each) and disp (3 each) are in/out as in the reference; vecs holds nVec (0..2)
+ * transported vectors per particle (3 doubles each), in/out. status (nullable) receives the number of face crossings
+ * of each particle, or a negative code when its walk failed (that particle keeps its input state); the call then
+ * returns CSG_ERR_WALK. */
+csg_status csg_transport_batch(csg_ctx* ctx, int32_t n, int32_t* face, double* bary, double* disp, int32_t nVec, double* vecs,
+                               int32_t* status);
+
+/* --- meshPositionToEuclideanLocation (src/models/triangulatedMeshSpace.cpp:82-106) ------------------------------- */
+csg_status csg_positions(csg_ctx* ctx, int32_t n, const int32_t* face, const double* bary, double* xyz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
