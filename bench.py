@@ -1,0 +1,156 @@
+"""Benchmark of the geodesic force path: MD steps/s and exact geodesic pairs/s on one MI355X.
+
+Workload (default): bumpy sphere L6 (81,920 faces), N = 20,000, truncated Gaussian (s = 0.05, cutoff 0.15),
+Nose-Hoover chain NVT at T = 0.5, dt = 0.005; one step = 2 walks + 1 neighbour search + 1 batched exact-geodesic force
+evaluation, all resident in HBM. Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1|2|3] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, face, bary, vel, threads, budget_s=30.0):
+    """The FP64 CPU restatement of the reference path (oracle, reference Release flags), timed on this host."""
+    from oracle import pyoracle
+
+    om = pyoracle.OracleMesh(w.X, w.T, fast=True)
+    P = (w.X[w.T[face]] * bary[:, :, None]).sum(1) / bary.sum(1)[:, None]
+    lo, hi = np.minimum(w.X.min(0), 0), np.maximum(w.X.max(0), 0)
+    if w.cell_list:
+        off, idx, md = pyoracle.cell_candidates(P, lo, hi, w.range, w.range)
+    else:
+        N = len(face)
+        off = np.arange(0, N * (N - 1) + 1, N - 1, dtype=np.int32)
+        idx = np.array([j for i in range(N) for j in range(N) if j != i], np.int32)
+        md = np.full(N, 1e20)
+    N = len(face)
+    kind = int(w.potential[0])
+    params = np.array(list(w.potential[1:]) + [0.0] * (3 - len(w.potential[1:])), float)
+    # probe cost on a few sources, then choose a sample that fits the budget
+    probe = min(N, max(threads * 4, 64))
+    sub_off = np.append(off[: probe + 1], np.full(N - probe, off[probe])).astype(np.int32)
+    t0 = time.perf_counter()
+    om.compute_forces(face, bary, sub_off, idx, md, kind, params, w.submeshing, w.maximum_distance, threads=threads,
+                      want_pairs=False)
+    per_src = (time.perf_counter() - t0) / probe
+    n_src = int(min(N, max(probe, budget_s / max(per_src, 1e-9))))
+    sub_off = np.append(off[: n_src + 1], np.full(N - n_src, off[n_src])).astype(np.int32)
+    t0 = time.perf_counter()
+    om.compute_forces(face, bary, sub_off, idx, md, kind, params, w.submeshing, w.maximum_distance, threads=threads,
+                      want_pairs=False)
+    t_force = (time.perf_counter() - t0) * N / n_src
+    disp = vel * (0.5 * w.dt)
+    t0 = time.perf_counter()
+    om.move(face, bary, disp, vel, threads=threads)
+    t_move = time.perf_counter() - t0
+    moves = 2 if w.integrator == "nvt" else 1
+    t_step = t_force + moves * t_move
+    return dict(value=1.0 / t_step, unit="steps/s", cores=threads, kind="port",
+                sample=(f"force evaluation of {n_src} of {N} sources (reference semantics: whole patch, no truncation) "
+                        f"scaled to {N}, plus {moves} full walk pass(es); oracle built -O3 -march=x86-64-v3 -ffast-math; "
+                        f"{threads} OpenMP threads"),
+                pairs_per_s=float(off[-1]) / t_step)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}); the multi-GPU design is described in "
+              "DESIGN.md but not built", file=sys.stderr)
+        sys.exit(2)
+
+    import curvedspacesim_amd as csa
+    from curvedspacesim_amd import workloads
+
+    w = workloads.config(args.config)
+    log(f"workload: {w.name}")
+    ctx = csa.Context(0)
+    sysm = workloads.make_system(ctx, w)
+    log(f"warmup {args.warmup} steps")
+    workloads.step(sysm, w, args.warmup)
+    sysm.sync()
+    st0 = sysm.status()
+    sysm.profile(True)
+    sysm.kernel_times()  # reset
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    workloads.step(sysm, w, args.steps)
+    sysm.sync()
+    elapsed = time.perf_counter() - t0
+    kt = sysm.kernel_times()
+    sysm.profile(False)
+    st = sysm.status()
+    ps = sysm.patch_stats()
+    if st["cand_overflow"] or st["unsolved"] or st["walk_failures"]:
+        raise SystemExit(f"step errors: {st}")
+    pairs = st["pairs"]
+    steps_s = args.steps / elapsed
+    forces_per_step = 1
+    # algorithmic bytes of one launch of the geodesic kernel: per source the patch's faces (3 corner ids + 3
+    # neighbour ids, 24 B), its vertices (FP64 xyz, 24 B), its targets (index + face + FP64 barycentrics, 32 B), the
+    # source's own state (28 B) and its force (24 B)
+    geo_ms, geo_n = kt["geodesic_lds"]
+    bytes_launch = 24.0 * ps["faces"] + 24.0 * ps["verts"] + 32.0 * pairs + 52.0 * len(w.face)
+    avg_ms = geo_ms / max(geo_n, 1)
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    out = {
+        "metric": "MD steps/s + exact geodesic pairs/s at N=20k; screened vs plain, same error",
+        "value": steps_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": w.name, "N": int(len(w.face)), "faces": int(len(w.T)), "cutoff": w.range,
+                   "integrator": w.integrator, "dt": w.dt, "screen": "plain (no learned screen)"},
+        "pairs_per_step": int(pairs) * forces_per_step,
+        "pairs_per_s": pairs * forces_per_step * steps_s,
+        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kt.items()},
+        "mean_patch_faces": ps["faces"] / len(w.face),
+        "max_patch_faces": ps["max_faces"],
+        "lds_fallbacks_per_step": {k: v / (args.steps + args.warmup) for k, v in sysm.fallback_counts().items()},
+        "roofline": {"kernel": "geodesicForceLds", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_launch},
+    }
+    if not args.no_cpu_baseline:
+        f, b, v, _ = sysm.get()
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline with {threads} threads")
+        out["cpu_baseline"] = cpu_baseline(w, f, b, v, threads)
+        out["speedup_vs_cpu"] = steps_s / out["cpu_baseline"]["value"]
+    else:
+        out["cpu_baseline"] = None
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

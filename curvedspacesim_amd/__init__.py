@@ -1,8 +1,8 @@
 """curvedSpaceSim geodesic force path on MI355X (gfx950): exact polyhedral geodesics, pair forces and the
 displacement walk as hand-written HIP kernels behind a C-ABI (include/csg.h), with a C++ host layer that keeps the
 reference's simulation/model/force/updater classes (include/csg_host.h, curvedspacesim_amd/csrc/host/)."""
-from ._native import (CsgError, Context, bumpy_sphere, icosphere, jittered_torus, read_off, HARMONIC, GAUSSIAN,
+from ._native import (CsgError, Context, System, bumpy_sphere, icosphere, jittered_torus, read_off, HARMONIC, GAUSSIAN,
                       TRUNC_GAUSSIAN, NO_POTENTIAL, PAIR_UNREACHED, PAIR_BEYOND, PAIR_SENTINEL)
 
-__all__ = ["CsgError", "Context", "bumpy_sphere", "icosphere", "jittered_torus", "read_off", "HARMONIC", "GAUSSIAN",
+__all__ = ["CsgError", "Context", "System", "bumpy_sphere", "icosphere", "jittered_torus", "read_off", "HARMONIC", "GAUSSIAN",
            "TRUNC_GAUSSIAN", "NO_POTENTIAL", "PAIR_UNREACHED", "PAIR_BEYOND", "PAIR_SENTINEL"]

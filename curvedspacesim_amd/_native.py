@@ -58,6 +58,21 @@ def csg():
                                   C.POINTER(csg_potential), C.c_int32, C.c_double, VP, VP, VP, VP, VP, VP]
     L.csg_transport_batch.argtypes = [VP, C.c_int32, _i32p, _f64p, _f64p, C.c_int32, VP, _i32p]
     L.csg_positions.argtypes = [VP, C.c_int32, _i32p, _f64p, _f64p]
+    L.csg_system_init.argtypes = [VP, C.c_int32, _i32p, _f64p, VP, VP]
+    L.csg_system_configure.argtypes = [VP, C.POINTER(csg_potential), C.c_double, C.c_int32, C.c_double, C.c_int32]
+    L.csg_system_compute_forces.argtypes = [VP, VP, VP]
+    L.csg_system_move.argtypes = [VP, _f64p, C.c_int32]
+    L.csg_system_nve.argtypes = [VP, C.c_double, C.c_int32]
+    L.csg_system_nvt.argtypes = [VP, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32]
+    L.csg_system_sync.argtypes = [VP]
+    L.csg_system_get.argtypes = [VP, VP, VP, VP, VP]
+    L.csg_system_status.argtypes = [VP, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+    L.csg_system_pairs.argtypes = [VP, VP, VP, C.c_int64]
+    L.csg_system_profile.argtypes = [VP, C.c_int32]
+    L.csg_system_kernel_times.argtypes = [VP, _f64p, _i32p]
+    L.csg_system_patch_stats.argtypes = [VP, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+    L.csg_system_fallback_counts.argtypes = [VP, _i32p]
     _csg = L
     return L
 
@@ -222,3 +237,85 @@ class Context:
         self._check(self.L.csg_positions(self.h, len(face), face, np.ascontiguousarray(bary, np.float64).ravel(),
                                          out.ravel()), "csg_positions")
         return out
+
+
+class System:
+    """Device-resident particle system of a Context (csg_system_* entry points)."""
+
+    def __init__(self, ctx, face, bary, vel=None, mass=None):
+        self.ctx = ctx
+        self.L = ctx.L
+        face = np.ascontiguousarray(face, np.int32)
+        self.N = len(face)
+        v = None if vel is None else np.ascontiguousarray(vel, np.float64).reshape(-1, 3)
+        m = None if mass is None else np.ascontiguousarray(mass, np.float64)
+        ctx._check(self.L.csg_system_init(ctx.h, self.N, face, np.ascontiguousarray(bary, np.float64).ravel(), _ptr(v),
+                                          _ptr(m)), "csg_system_init")
+
+    def configure(self, potential, range_, cell_list=True, rmax=np.inf, max_patch_faces=0):
+        pot = csg_potential(int(potential[0]), *[float(x) for x in (list(potential[1:]) + [0, 0, 0])[:3]])
+        rm = float(rmax) if np.isfinite(rmax) else 1e300
+        self.ctx._check(self.L.csg_system_configure(self.ctx.h, C.byref(pot), float(range_), int(bool(cell_list)), rm,
+                                                    int(max_patch_faces)), "csg_system_configure")
+
+    def compute_forces(self):
+        F = np.zeros((self.N, 3))
+        e = C.c_double()
+        self.ctx._check(self.L.csg_system_compute_forces(self.ctx.h, _ptr(F), C.byref(e)), "csg_system_compute_forces")
+        return F, e.value
+
+    def move(self, disp, transport_velocity=True):
+        self.ctx._check(self.L.csg_system_move(self.ctx.h, np.ascontiguousarray(disp, np.float64).ravel(),
+                                               int(bool(transport_velocity))), "csg_system_move")
+
+    def nve(self, dt, steps):
+        self.ctx._check(self.L.csg_system_nve(self.ctx.h, float(dt), int(steps)), "csg_system_nve")
+
+    def nvt(self, dt, T, tau=1.0, M=2, steps=1):
+        self.ctx._check(self.L.csg_system_nvt(self.ctx.h, float(dt), float(T), float(tau), int(M), int(steps)),
+                        "csg_system_nvt")
+
+    def sync(self):
+        self.ctx._check(self.L.csg_system_sync(self.ctx.h), "csg_system_sync")
+
+    def get(self):
+        f = np.zeros(self.N, np.int32)
+        b = np.zeros((self.N, 3))
+        v = np.zeros((self.N, 3))
+        F = np.zeros((self.N, 3))
+        self.ctx._check(self.L.csg_system_get(self.ctx.h, _ptr(f), _ptr(b), _ptr(v), _ptr(F)), "csg_system_get")
+        return f, b, v, F
+
+    def status(self):
+        p, o, u, w, k = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+        self.ctx._check(self.L.csg_system_status(self.ctx.h, C.byref(p), C.byref(o), C.byref(u), C.byref(w), C.byref(k)),
+                        "csg_system_status")
+        return dict(pairs=p.value, cand_overflow=o.value, unsolved=u.value, walk_failures=w.value, kinetic=k.value)
+
+    def pairs(self):
+        off = np.zeros(self.N + 1, np.int32)
+        self.ctx._check(self.L.csg_system_pairs(self.ctx.h, _ptr(off), None, 0), "csg_system_pairs")
+        idx = np.zeros(max(int(off[-1]), 1), np.int32)
+        self.ctx._check(self.L.csg_system_pairs(self.ctx.h, _ptr(off), _ptr(idx), len(idx)), "csg_system_pairs")
+        return off, idx[: off[-1]]
+
+    def profile(self, enable=True):
+        self.ctx._check(self.L.csg_system_profile(self.ctx.h, int(bool(enable))), "csg_system_profile")
+
+    def kernel_times(self):
+        ms = np.zeros(4)
+        n = np.zeros(4, np.int32)
+        self.ctx._check(self.L.csg_system_kernel_times(self.ctx.h, ms, n), "csg_system_kernel_times")
+        names = ["neighbors", "geodesic_lds", "geodesic_hbm", "transport"]
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(names)}
+
+    def fallback_counts(self):
+        r = np.zeros(6, np.int32)
+        self.ctx._check(self.L.csg_system_fallback_counts(self.ctx.h, r), "csg_system_fallback_counts")
+        return dict(zip(["faces", "verts", "queue", "targets", "steps", "frontier"], r.tolist()))
+
+    def patch_stats(self):
+        f, v, m = C.c_int64(), C.c_int64(), C.c_int32()
+        self.ctx._check(self.L.csg_system_patch_stats(self.ctx.h, C.byref(f), C.byref(v), C.byref(m)),
+                        "csg_system_patch_stats")
+        return dict(faces=f.value, verts=v.value, max_faces=m.value)

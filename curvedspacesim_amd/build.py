@@ -8,7 +8,7 @@ CSRC = os.path.join(PKG, "csrc")
 ROOT = os.path.dirname(PKG)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["csg_api.hip", "geodesic_kernel.hip", "transport_kernel.hip"]
+HIP_SOURCES = ["csg_api.hip", "csg_system.hip", "geodesic_kernel.hip", "transport_kernel.hip"]
 HOST_SOURCES = ["host/meshGenerators.cpp", "host/hostCapi.cpp"]
 # packed FP32 instructions are kept out of every code object (see the hipcc feature flag below)
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -10,76 +10,14 @@
 
 #include "../../include/csg.h"
 #include "csg_common.h"
+#include "csg_context.h"
 #include "geodesic_kernel.h"
 #include "transport_kernel.h"
 
 namespace {
 
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    hipError_t ensure(size_t n) {
-        if (n <= bytes && p) return hipSuccess;
-        if (p) hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        size_t want = std::max<size_t>(n, 256);
-        hipError_t e = hipMalloc(&p, want);
-        if (e == hipSuccess) bytes = want;
-        return e;
-    }
-    void release() {
-        if (p) hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <class T>
-    T* as() const {
-        return (T*)p;
-    }
-};
-
-}  // namespace
-
-struct csg_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    std::string arch;
-    // mesh
-    int nV = 0, nF = 0;
-    double area = 0, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    DevBuf vx, fv, fadj, fgeo, fnrm, vflag, vfoff, vflist;
-    csg::DeviceMesh dm{};
-    bool haveMesh = false;
-    int submeshing = 0;
-    double maximumDistance = 0;
-    // per-call staging
-    DevBuf pface, pbary, srcIdx, toff, tidx, prad, pdist, pst, pet, pflags, pforces, penergy, status, list;
-    DevBuf tdisp, tvecs, tstat;
-    DevBuf scratch;
-    int scratchBlocks = 0;
-    csg::ScratchLayout layout{};
-    ~csg_ctx() {
-        DevBuf* all[] = {&vx, &fv, &fadj, &fgeo, &fnrm, &vflag, &vfoff, &vflist, &pface, &pbary, &srcIdx, &toff, &tidx, &prad,
-                         &pdist, &pst, &pet, &pflags, &pforces, &penergy, &status, &list, &tdisp, &tvecs, &tstat, &scratch};
-        for (DevBuf* b : all) b->release();
-        if (stream) hipStreamDestroy(stream);
-    }
-};
-
-namespace {
-
-csg_status fail(csg_ctx* c, csg_status s, const std::string& msg) {
-    if (c) c->err = msg;
-    return s;
-}
-
-#define HIPCHECK(ctx, expr)                                                                                  \
-    do {                                                                                                     \
-        hipError_t e_ = (expr);                                                                              \
-        if (e_ != hipSuccess) return fail(ctx, CSG_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
-    } while (0)
+using csg::fail;
+using csg::DevBuf;
 
 template <class T>
 hipError_t upload(csg_ctx* c, DevBuf& b, const T* host, size_t n) {

@@ -1,0 +1,87 @@
+// Internal definition of the C-ABI context (shared by the translation units of libcsg.so).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "../../include/csg.h"
+#include "csg_common.h"
+#include "geodesic_kernel.h"
+
+namespace csg {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= bytes && p) return hipSuccess;
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t want = std::max<size_t>(n, 256);
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) bytes = want;
+        return e;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const {
+        return (T*)p;
+    }
+};
+
+struct SystemState;  // device-resident particle system (csg_system.hip)
+void destroySystem(SystemState* s);
+
+}  // namespace csg
+
+struct csg_ctx {
+    using DevBuf = csg::DevBuf;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::string arch;
+    // mesh
+    int nV = 0, nF = 0;
+    double area = 0, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    DevBuf vx, fv, fadj, fgeo, fnrm, vflag, vfoff, vflist;
+    csg::DeviceMesh dm{};
+    bool haveMesh = false;
+    int submeshing = 0;
+    double maximumDistance = 0;
+    // per-call staging
+    DevBuf pface, pbary, srcIdx, toff, tidx, prad, pdist, pst, pet, pflags, pforces, penergy, status, list;
+    DevBuf tdisp, tvecs, tstat;
+    DevBuf scratch;
+    int scratchBlocks = 0;
+    csg::ScratchLayout layout{};
+    csg::SystemState* sys = nullptr;
+    ~csg_ctx() {
+        if (sys) csg::destroySystem(sys);
+        DevBuf* all[] = {&vx, &fv, &fadj, &fgeo, &fnrm, &vflag, &vfoff, &vflist, &pface, &pbary, &srcIdx, &toff, &tidx, &prad,
+                         &pdist, &pst, &pet, &pflags, &pforces, &penergy, &status, &list, &tdisp, &tvecs, &tstat, &scratch};
+        for (DevBuf* b : all) b->release();
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+namespace csg {
+
+inline csg_status fail(csg_ctx* c, csg_status s, const std::string& msg) {
+    if (c) c->err = msg;
+    return s;
+}
+
+#define HIPCHECK(ctx, expr)                                                                                      \
+    do {                                                                                                         \
+        hipError_t e_ = (expr);                                                                                  \
+        if (e_ != hipSuccess) return csg::fail(ctx, CSG_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace csg

@@ -1,0 +1,789 @@
+// Device-resident particle system: the batched form of simpleModel (positions, velocities, forces in HBM), its
+// neighbour search, the fused force computation, and the velocity-Verlet and Nose-Hoover-chain integrators.
+//
+//   cell list      src/utility/hyperRectangularCellList.cpp:9-159, src/utility/cellListNeighborStructure.cpp:22-84
+//                  (grid of floor(extent/range) cells per axis over the box seeded at the origin; candidates from the
+//                  27 surrounding cells, x outermost then y then z, inside a cell by particle index; |xi-xj|^2 < r^2)
+//   all-to-all     src/utility/baseNeighborStructure.cpp:17-36 (every other particle by index; threshold 1e20)
+//   forces         src/models/simpleModel.cpp:68-112 + src/forces/baseForce.cpp:12-28, fused (geodesic_kernel.hip)
+//   move           src/models/simpleModel.cpp:44-66 (velocity transported) via transport_kernel.hip
+//   VV-NVE         src/updaters/velocityVerletNVE.cpp:3-29
+//   NH-NVT         src/updaters/noseHooverNVT.cpp:3-150 (chain of M thermostats, tau, Q0 = 2(N-1) T tau^2, Qi = T tau^2;
+//                  kinetic energy initialised to Q0 as the reference does)
+// Every step is a fixed sequence of kernel launches on the context's stream with no host synchronisation; error
+// conditions (candidate capacity, unsolved sources, failed walks) are counted on the device and read by
+// csg_system_status().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/csg.h"
+#include "csg_common.h"
+#include "csg_context.h"
+#include "geodesic_kernel.h"
+#include "transport_kernel.h"
+
+namespace csg {
+
+// device counters; CNT_LDS_REASON + b counts sources the LDS variant handed to the HBM variant for OverflowReason bit b
+enum Counter : int { CNT_CAND_OVERFLOW = 0, CNT_UNSOLVED = 1, CNT_WALK_FAIL = 2, CNT_PAIRS = 3, CNT_LDS_REASON = 4, CNT_N = 10 };
+
+struct SystemState {
+    int N = 0;
+    DevBuf face, bary, vel, force, mass, energy;
+    DevBuf xyz, key, cellCount, cellStart, cellFill, order;
+    DevBuf cnt, off, toff, cand, maxd, rad;
+    DevBuf disp, tstatus, gstatus, partial, counters, ke, chain, pstat;
+    DevBuf scratch;
+    ScratchLayout layout{};
+    int scratchBlocks = 0;
+    // configuration
+    Potential pot{POT_NONE, 0, 0, 0};
+    double range = 1.0, rMax = HUGE_VAL;
+    int cellList = 0;
+    int n3[3] = {1, 1, 1};
+    double cellSize[3] = {1, 1, 1};
+    int nCells = 1;
+    long candCap = 0;
+    bool configured = false;
+    // Nose-Hoover chain
+    int M = 0;
+    double T = 0, tau = 1;
+    bool chainReady = false;
+    // optional event timing of the launches, by kind
+    bool profiling = false;
+    struct Mark {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Mark> marks;
+};
+
+enum MarkKind : int { MARK_NEIGHBORS = 0, MARK_GEODESIC_LDS = 1, MARK_GEODESIC_HBM = 2, MARK_TRANSPORT = 3, MARK_N = 4 };
+
+// records an event pair around a group of launches on the context stream when profiling is on
+struct MarkScope {
+    SystemState* S;
+    hipStream_t st;
+    int kind;
+    hipEvent_t a = nullptr, b = nullptr;
+    MarkScope(SystemState* s, hipStream_t stream, int k) : S(s), st(stream), kind(k) {
+        if (!S->profiling) return;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        hipEventRecord(a, st);
+    }
+    ~MarkScope() {
+        if (!S->profiling) return;
+        hipEventRecord(b, st);
+        S->marks.push_back({kind, a, b});
+    }
+};
+
+void destroySystem(SystemState* s) {
+    DevBuf* all[] = {&s->face, &s->bary, &s->vel, &s->force, &s->mass, &s->energy, &s->xyz, &s->key, &s->cellCount, &s->cellStart,
+                     &s->cellFill, &s->order, &s->cnt, &s->off, &s->toff, &s->cand, &s->maxd, &s->rad, &s->disp, &s->tstatus,
+                     &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->scratch};
+    for (DevBuf* b : all) b->release();
+    for (auto& m : s->marks) {
+        hipEventDestroy(m.a);
+        hipEventDestroy(m.b);
+    }
+    delete s;
+}
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline int blocksFor(long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+__global__ void kFillInt(int* p, int v, long n) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void kPositions(DeviceMesh M, int N, const int32_t* face, const double* bary, double* xyz) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int* c = M.fv + 3 * face[i];
+    const double* b = bary + 3 * i;
+    double s = b[0] + b[1] + b[2];
+    for (int a = 0; a < 3; ++a) xyz[3 * i + a] = (M.vx[3 * c[0] + a] * b[0] + M.vx[3 * c[1] + a] * b[1] + M.vx[3 * c[2] + a] * b[2]) / s;
+}
+
+struct Grid {
+    double lo[3], size[3];
+    int n[3];
+};
+
+__device__ inline int cellCoord(const Grid& g, const double* p, int a) {
+    int v = (int)floor((p[a] - g.lo[a]) / g.size[a]);
+    return max(0, min(g.n[a] - 1, v));
+}
+
+__global__ void kCellKeys(Grid g, int N, const double* xyz, int* key, int* cellCount) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double* p = xyz + 3 * i;
+    int c = cellCoord(g, p, 0) + g.n[0] * (cellCoord(g, p, 1) + g.n[1] * cellCoord(g, p, 2));
+    key[i] = c;
+    atomicAdd(&cellCount[c], 1);
+}
+
+// exclusive prefix sum of n ints into out[0..n] (out[n] = total), one workgroup of 1024 lanes
+__global__ __launch_bounds__(1024) void kScanExclusive(const int* in, int* out, int n) {
+    __shared__ int buf[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        int i = base + threadIdx.x;
+        int v = i < n ? in[i] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            int t = threadIdx.x >= o ? buf[threadIdx.x - o] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < n) out[i] = carry + buf[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += buf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ void kScatter(int N, const int* key, const int* cellStart, int* cellFill, int* order) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int c = key[i];
+    order[cellStart[c] + atomicAdd(&cellFill[c], 1)] = i;
+}
+
+// particles of a cell in increasing index (the reference fills cells in particle order)
+__global__ void kSortCells(int nCells, const int* cellStart, int* order) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nCells) return;
+    int b = cellStart[c], e = cellStart[c + 1];
+    for (int i = b + 1; i < e; ++i) {
+        int v = order[i], j = i - 1;
+        while (j >= b && order[j] > v) {
+            order[j + 1] = order[j];
+            --j;
+        }
+        order[j + 1] = v;
+    }
+}
+
+template <bool kFill>
+__global__ void kCandidates(Grid g, int N, const double* xyz, const int* key, const int* cellStart, const int* order, double r2, int* cnt,
+                            const int* off, long cap, int* cand, double* maxd, double* rad, double maxDist, int* counters) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double* p = xyz + 3 * i;
+    int c = key[i];
+    int cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / (g.n[0] * g.n[1]);
+    int k = 0;
+    double m2 = 0;
+    long base = kFill ? off[i] : 0;
+    for (int x = max(0, cx - 1); x <= min(g.n[0] - 1, cx + 1); ++x)
+        for (int y = max(0, cy - 1); y <= min(g.n[1] - 1, cy + 1); ++y)
+            for (int z = max(0, cz - 1); z <= min(g.n[2] - 1, cz + 1); ++z) {
+                int cc = x + g.n[0] * (y + g.n[1] * z);
+                for (int q = cellStart[cc]; q < cellStart[cc + 1]; ++q) {
+                    int j = order[q];
+                    if (j == i) continue;
+                    double dx = p[0] - xyz[3 * j], dy = p[1] - xyz[3 * j + 1], dz = p[2] - xyz[3 * j + 2];
+                    double d2 = dx * dx + dy * dy + dz * dz;
+                    if (d2 < r2) {
+                        if (kFill && base + k < cap) cand[base + k] = j;
+                        ++k;
+                        if (d2 > m2) m2 = d2;
+                    }
+                }
+            }
+    if (!kFill) {
+        cnt[i] = k;
+    } else {
+        if (base + k > cap) atomicAdd(&counters[CNT_CAND_OVERFLOW], 1);
+        double md = sqrt(m2);
+        maxd[i] = md;
+        rad[i] = md < maxDist ? md : maxDist;
+    }
+}
+
+// all-to-all candidates: every other particle by increasing index
+__global__ void kAllToAll(int N, int* cand, int* off, double* maxd, double* rad, double maxDist) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > N) return;
+    off[i] = i * (N - 1);
+    if (i == N) return;
+    int k = 0;
+    for (int j = 0; j < N; ++j)
+        if (j != i) cand[(long)i * (N - 1) + k++] = j;
+    maxd[i] = 1e20;
+    rad[i] = 1e20 < maxDist ? 1e20 : maxDist;
+}
+
+// offsets clamped to the candidate capacity, so a source never reads past it (its step is then flagged invalid)
+__global__ void kClampOffsets(int N, const int* off, long cap, int* toff, int* counters) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > N) return;
+    toff[i] = (int)min((long)off[i], cap);
+    if (i == N) counters[CNT_PAIRS] = off[N];
+}
+
+__global__ void kCountReasons(int N, const int* status, int* counters) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int s = status[i];
+    for (int b = 0; b < 6; ++b)
+        if (s & (1 << b)) atomicAdd(&counters[CNT_LDS_REASON + b], 1);
+}
+
+__global__ void kCountNonzero(int N, const int* v, int* counters, int which, bool negativeOnly) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    bool bad = negativeOnly ? v[i] < 0 : v[i] != 0;
+    if (bad) atomicAdd(&counters[which], 1);
+}
+
+// velocityVerletNVE first half: displacement and half kick
+__global__ void kVVFirst(int N, double dt, double* v, const double* f, double* disp) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    for (int a = 0; a < 3; ++a) {
+        disp[3 * i + a] = dt * v[3 * i + a] + 0.5 * dt * dt * f[3 * i + a];
+        v[3 * i + a] += 0.5 * dt * f[3 * i + a];
+    }
+}
+
+__global__ void kVVKick(int N, double dt, double* v, const double* f) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    for (int a = 0; a < 3; ++a) v[3 * i + a] += 0.5 * dt * f[3 * i + a];
+}
+
+__global__ void kScaledCopy(int N, double s, const double* v, double* disp) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * N) return;
+    disp[i] = s * v[i];
+}
+
+// noseHooverNVT::propagatePositionsVelocities second half: kick, next displacement, per-block kinetic energy
+__global__ __launch_bounds__(256) void kNHKick(int N, double dt, double dt2, double* v, const double* f, const double* m, double* disp,
+                                               double* partial) {
+    __shared__ double red[256];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double ke = 0;
+    if (i < N) {
+        double mi = m[i];
+        double vv = 0;
+        for (int a = 0; a < 3; ++a) {
+            double x = v[3 * i + a] + (dt / mi) * f[3 * i + a];
+            v[3 * i + a] = x;
+            disp[3 * i + a] = dt2 * x;
+            vv += x * x;
+        }
+        ke = 0.5 * mi * vv;
+    }
+    red[threadIdx.x] = ke;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// fixed-order sum of the per-block partials (deterministic)
+__global__ void kSumPartials(int n, const double* partial, double* ke) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0;
+    for (int i = 0; i < n; ++i) s += partial[i];
+    ke[0] = s;
+}
+
+// noseHooverNVT::propagateChain; chain holds (x, y, z, w) = (position, velocity, acceleration, mass) for M+1 entries,
+// ke[0] the kinetic energy, ke[1] receives the velocity scale factor
+__global__ void kNHChain(double* chain, double* ke, int M, double T, double dt) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double dt2 = 0.5 * dt, dt4 = 0.25 * dt, dt8 = 0.125 * dt;
+    double4* b = (double4*)chain;
+    double K = ke[0];
+    for (int ii = M - 1; ii > 0; --ii) {
+        b[ii].z = (b[ii - 1].w * b[ii - 1].y * b[ii - 1].y - T) / b[ii].w;
+        double ef = exp(-dt8 * b[ii + 1].y);
+        b[ii].y *= ef;
+        b[ii].y += b[ii].z * dt4;
+        b[ii].y *= ef;
+    }
+    b[0].z = (2.0 * K / b[0].w - 1.0);
+    double ef = exp(-dt8 * b[1].y);
+    b[0].y *= ef;
+    b[0].y += b[0].z * dt4;
+    b[0].y *= ef;
+    for (int ii = 0; ii < M; ++ii) b[ii].x += dt2 * b[ii].y;
+    double s = exp(-dt2 * b[0].y);
+    K = s * s * K;
+    b[0].z = (2.0 * K / b[0].w - 1.0);
+    ef = exp(-dt8 * b[1].y);
+    b[0].y *= ef;
+    b[0].y += b[0].z * dt4;
+    b[0].y *= ef;
+    for (int ii = 1; ii < M; ++ii) {
+        b[ii].z = (b[ii - 1].w * b[ii - 1].y * b[ii - 1].y - T) / b[ii].w;
+        ef = exp(-dt8 * b[ii + 1].y);
+        b[ii].y *= ef;
+        b[ii].y += b[ii].z * dt4;
+        b[ii].y *= ef;
+    }
+    ke[0] = K;
+    ke[1] = s;
+}
+
+__global__ void kScaleVel(int N, double* v, const double* ke) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * N) return;
+    v[i] *= ke[1];
+}
+
+}  // namespace
+
+// ---- step pieces (all asynchronous on the context stream) ------------------------------------------------------
+
+static csg_status sysNeighbors(csg_ctx* c, SystemState* S) {
+    hipStream_t st = c->stream;
+    MarkScope mark(S, st, MARK_NEIGHBORS);
+    const int N = S->N;
+    if (!S->cellList) {
+        hipLaunchKernelGGL(kAllToAll, dim3(blocksFor(N + 1)), dim3(kBlock), 0, st, N, S->cand.as<int>(), S->off.as<int>(),
+                           S->maxd.as<double>(), S->rad.as<double>(), c->maximumDistance);
+        hipLaunchKernelGGL(kClampOffsets, dim3(blocksFor(N + 1)), dim3(kBlock), 0, st, N, S->off.as<int>(), S->candCap, S->toff.as<int>(),
+                           S->counters.as<int>());
+        HIPCHECK(c, hipGetLastError());
+        return CSG_OK;
+    }
+    Grid g;
+    for (int a = 0; a < 3; ++a) {
+        g.lo[a] = c->lo[a];
+        g.size[a] = S->cellSize[a];
+        g.n[a] = S->n3[a];
+    }
+    hipLaunchKernelGGL(kPositions, dim3(blocksFor(N)), dim3(kBlock), 0, st, c->dm, N, S->face.as<int32_t>(), S->bary.as<double>(),
+                       S->xyz.as<double>());
+    hipLaunchKernelGGL(kFillInt, dim3(blocksFor(S->nCells)), dim3(kBlock), 0, st, S->cellCount.as<int>(), 0, (long)S->nCells);
+    hipLaunchKernelGGL(kFillInt, dim3(blocksFor(S->nCells)), dim3(kBlock), 0, st, S->cellFill.as<int>(), 0, (long)S->nCells);
+    hipLaunchKernelGGL(kCellKeys, dim3(blocksFor(N)), dim3(kBlock), 0, st, g, N, S->xyz.as<double>(), S->key.as<int>(),
+                       S->cellCount.as<int>());
+    hipLaunchKernelGGL(kScanExclusive, dim3(1), dim3(1024), 0, st, S->cellCount.as<int>(), S->cellStart.as<int>(), S->nCells);
+    hipLaunchKernelGGL(kScatter, dim3(blocksFor(N)), dim3(kBlock), 0, st, N, S->key.as<int>(), S->cellStart.as<int>(), S->cellFill.as<int>(),
+                       S->order.as<int>());
+    hipLaunchKernelGGL(kSortCells, dim3(blocksFor(S->nCells)), dim3(kBlock), 0, st, S->nCells, S->cellStart.as<int>(), S->order.as<int>());
+    double r2 = S->range * S->range;
+    hipLaunchKernelGGL((kCandidates<false>), dim3(blocksFor(N)), dim3(kBlock), 0, st, g, N, S->xyz.as<double>(), S->key.as<int>(),
+                       S->cellStart.as<int>(), S->order.as<int>(), r2, S->cnt.as<int>(), (const int*)nullptr, S->candCap, (int*)nullptr,
+                       (double*)nullptr, (double*)nullptr, c->maximumDistance, S->counters.as<int>());
+    hipLaunchKernelGGL(kScanExclusive, dim3(1), dim3(1024), 0, st, S->cnt.as<int>(), S->off.as<int>(), N);
+    hipLaunchKernelGGL((kCandidates<true>), dim3(blocksFor(N)), dim3(kBlock), 0, st, g, N, S->xyz.as<double>(), S->key.as<int>(),
+                       S->cellStart.as<int>(), S->order.as<int>(), r2, S->cnt.as<int>(), S->off.as<int>(), S->candCap, S->cand.as<int>(),
+                       S->maxd.as<double>(), S->rad.as<double>(), c->maximumDistance, S->counters.as<int>());
+    hipLaunchKernelGGL(kClampOffsets, dim3(blocksFor(N + 1)), dim3(kBlock), 0, st, N, S->off.as<int>(), S->candCap, S->toff.as<int>(),
+                       S->counters.as<int>());
+    HIPCHECK(c, hipGetLastError());
+    return CSG_OK;
+}
+
+static csg_status sysForces(csg_ctx* c, SystemState* S) {
+    csg_status s = sysNeighbors(c, S);
+    if (s) return s;
+    GeoParams G{};
+    G.mesh = c->dm;
+    G.pface = S->face.as<int32_t>();
+    G.pbary = S->bary.as<double>();
+    G.nSrc = S->N;
+    G.srcIdx = nullptr;
+    G.toff = S->toff.as<int32_t>();
+    G.tidx = S->cand.as<int32_t>();
+    G.patchRadius = c->submeshing ? S->rad.as<double>() : nullptr;
+    G.rMax = S->rMax;
+    G.forces = S->force.as<double>();
+    G.energy = S->energy.as<double>();
+    G.status = S->gstatus.as<int>();
+    G.pot = S->pot;
+    G.useSentinel = c->submeshing;
+    G.sentinelDist = 2.0 * c->maximumDistance;
+    G.patchStats = S->pstat.as<int>();
+    if (c->submeshing) {
+        {
+            MarkScope mark(S, c->stream, MARK_GEODESIC_LDS);
+            HIPCHECK(c, launchGeodesicLds(G, c->stream));
+        }
+        hipLaunchKernelGGL(kCountReasons, dim3(blocksFor(S->N)), dim3(kBlock), 0, c->stream, S->N, S->gstatus.as<int>(), S->counters.as<int>());
+        MarkScope mark(S, c->stream, MARK_GEODESIC_HBM);
+        HIPCHECK(c, launchGeodesicScratchRedo(G, S->layout, S->scratchBlocks, c->stream));
+    } else {
+        MarkScope mark(S, c->stream, MARK_GEODESIC_HBM);
+        HIPCHECK(c, launchGeodesicScratch(G, S->layout, nullptr, S->N, S->scratchBlocks, c->stream));
+    }
+    hipLaunchKernelGGL(kCountNonzero, dim3(blocksFor(S->N)), dim3(kBlock), 0, c->stream, S->N, S->gstatus.as<int>(), S->counters.as<int>(),
+                       (int)CNT_UNSOLVED, false);
+    HIPCHECK(c, hipGetLastError());
+    return CSG_OK;
+}
+
+static csg_status sysMove(csg_ctx* c, SystemState* S) {
+    TransportParams T{};
+    T.mesh = c->dm;
+    T.n = S->N;
+    T.face = S->face.as<int32_t>();
+    T.bary = S->bary.as<double>();
+    T.disp = S->disp.as<double>();
+    T.scale = 1.0;
+    T.dispOut = nullptr;
+    T.nVec = 1;
+    T.vecs = S->vel.as<double>();
+    T.status = S->tstatus.as<int32_t>();
+    T.maxCrossings = 100000;
+    {
+        MarkScope mark(S, c->stream, MARK_TRANSPORT);
+        HIPCHECK(c, launchTransport(T, c->stream));
+    }
+    hipLaunchKernelGGL(kCountNonzero, dim3(blocksFor(S->N)), dim3(kBlock), 0, c->stream, S->N, S->tstatus.as<int>(), S->counters.as<int>(),
+                       (int)CNT_WALK_FAIL, true);
+    HIPCHECK(c, hipGetLastError());
+    return CSG_OK;
+}
+
+}  // namespace csg
+
+using namespace csg;
+
+static SystemState* sysOf(csg_ctx* c) { return c ? c->sys : nullptr; }
+
+extern "C" {
+
+csg_status csg_system_init(csg_ctx* c, int32_t N, const int32_t* face, const double* bary, const double* vel, const double* mass) {
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!c->haveMesh) return fail(c, CSG_ERR_STATE, "no mesh uploaded");
+    if (N <= 1 || !face || !bary) return fail(c, CSG_ERR_ARGUMENT, "need at least two particles with positions");
+    for (int i = 0; i < N; ++i)
+        if (face[i] < 0 || face[i] >= c->nF) return fail(c, CSG_ERR_ARGUMENT, "particle face out of range");
+    HIPCHECK(c, hipSetDevice(c->device));
+    if (!c->sys) c->sys = new SystemState();
+    SystemState* S = c->sys;
+    S->N = N;
+    S->configured = false;
+    S->chainReady = false;
+    std::vector<double> zeros(3 * (size_t)N, 0.0), ones(N, 1.0);
+    HIPCHECK(c, S->face.ensure(4 * (size_t)N));
+    HIPCHECK(c, S->bary.ensure(24 * (size_t)N));
+    HIPCHECK(c, S->vel.ensure(24 * (size_t)N));
+    HIPCHECK(c, S->force.ensure(24 * (size_t)N));
+    HIPCHECK(c, S->mass.ensure(8 * (size_t)N));
+    HIPCHECK(c, hipMemcpyAsync(S->face.p, face, 4 * (size_t)N, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(S->bary.p, bary, 24 * (size_t)N, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(S->vel.p, vel ? vel : zeros.data(), 24 * (size_t)N, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(S->force.p, zeros.data(), 24 * (size_t)N, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(S->mass.p, mass ? mass : ones.data(), 8 * (size_t)N, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, S->energy.ensure(8 * (size_t)N));
+    HIPCHECK(c, S->xyz.ensure(24 * (size_t)N));
+    HIPCHECK(c, S->key.ensure(4 * (size_t)N));
+    HIPCHECK(c, S->order.ensure(4 * (size_t)N));
+    HIPCHECK(c, S->cnt.ensure(4 * ((size_t)N + 1)));
+    HIPCHECK(c, S->off.ensure(4 * ((size_t)N + 1)));
+    HIPCHECK(c, S->toff.ensure(4 * ((size_t)N + 1)));
+    HIPCHECK(c, S->maxd.ensure(8 * (size_t)N));
+    HIPCHECK(c, S->rad.ensure(8 * (size_t)N));
+    HIPCHECK(c, S->disp.ensure(24 * (size_t)N));
+    HIPCHECK(c, S->tstatus.ensure(4 * (size_t)N));
+    HIPCHECK(c, S->gstatus.ensure(4 * (size_t)N));
+    HIPCHECK(c, S->pstat.ensure(8 * (size_t)N));
+    HIPCHECK(c, S->partial.ensure(8 * (size_t)blocksFor(N)));
+    HIPCHECK(c, S->counters.ensure(4 * CNT_N));
+    HIPCHECK(c, S->ke.ensure(16));
+    HIPCHECK(c, hipMemsetAsync(S->counters.p, 0, 4 * CNT_N, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return CSG_OK;
+}
+
+csg_status csg_system_configure(csg_ctx* c, const csg_potential* pot, double range, int32_t cellList, double rMax, int32_t maxPatchFaces) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
+    if (!(range > 0)) return fail(c, CSG_ERR_ARGUMENT, "range must be positive");
+    HIPCHECK(c, hipSetDevice(c->device));
+    S->pot = pot ? Potential{pot->kind, pot->p0, pot->p1, pot->p2} : Potential{POT_NONE, 0, 0, 0};
+    S->range = range;
+    S->cellList = cellList ? 1 : 0;
+    S->rMax = rMax;
+    const int N = S->N;
+    if (S->cellList) {
+        long cells = 1;
+        for (int a = 0; a < 3; ++a) {
+            double ext = c->hi[a] - c->lo[a];
+            S->n3[a] = std::max(1, (int)std::floor(ext / range));
+            S->cellSize[a] = ext / S->n3[a];
+            cells *= S->n3[a];
+        }
+        if (cells > (1L << 24)) return fail(c, CSG_ERR_CAPACITY, "more than 2^24 cells: interaction range too small for the mesh box");
+        S->nCells = (int)cells;
+        HIPCHECK(c, S->cellCount.ensure(4 * (size_t)cells));
+        HIPCHECK(c, S->cellFill.ensure(4 * (size_t)cells));
+        HIPCHECK(c, S->cellStart.ensure(4 * ((size_t)cells + 1)));
+        // capacity: 8x the mean expected candidates plus 64 per particle
+        double dens = N / std::max(c->area, 1e-300);
+        long per = (long)(8.0 * dens * M_PI * range * range) + 64;
+        S->candCap = std::min<long>((long)N * std::min<long>(per, N - 1), (1L << 31) - 1);
+    } else {
+        S->candCap = (long)N * (N - 1);
+        if (S->candCap >= (1L << 31)) return fail(c, CSG_ERR_CAPACITY, "all-to-all pair count exceeds 2^31");
+    }
+    HIPCHECK(c, S->cand.ensure(4 * (size_t)std::max<long>(S->candCap, 1)));
+    // HBM scratch for sources the LDS tables cannot hold, or for whole-mesh solves
+    // sources that outgrow the LDS tables are rare (a few per force evaluation); many narrow workgroups take them
+    long faces = c->submeshing ? std::max<int>(maxPatchFaces > 0 ? maxPatchFaces : 1536, 1024) : c->nF;
+    long maxT = c->submeshing && S->cellList ? std::min<long>(S->candCap, 4096) : N;
+    int blocks = c->submeshing ? 1024 : std::min(N, 256);
+    ScratchLayout L{};
+    auto p2 = [](long n) {
+        int p = 1;
+        while (p < n) p <<= 1;
+        return p;
+    };
+    L.fcap = p2(std::max(2 * faces + 2, 1024L));
+    L.vcap = p2(std::max(2 * faces + 16, 512L));
+    L.qcap = std::max(p2(std::max(16 * faces, 4096L)), L.fcap);
+    L.tcap = (int)std::max(maxT, 1L);
+    L.bytesPerBlock = scratchBytesPerBlock(L.fcap, L.vcap, L.qcap, L.tcap);
+    HIPCHECK(c, S->scratch.ensure(L.bytesPerBlock * (size_t)blocks));
+    L.base = S->scratch.p;
+    S->layout = L;
+    S->scratchBlocks = blocks;
+    S->configured = true;
+    return CSG_OK;
+}
+
+csg_status csg_system_compute_forces(csg_ctx* c, double* forces, double* energy) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S || !S->configured) return fail(c, CSG_ERR_STATE, "csg_system_init and csg_system_configure first");
+    HIPCHECK(c, hipSetDevice(c->device));
+    csg_status s = sysForces(c, S);
+    if (s) return s;
+    if (forces) HIPCHECK(c, hipMemcpyAsync(forces, S->force.p, 24 * (size_t)S->N, hipMemcpyDeviceToHost, c->stream));
+    if (energy) {
+        std::vector<double> e(S->N);
+        HIPCHECK(c, hipMemcpyAsync(e.data(), S->energy.p, 8 * (size_t)S->N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        double t = 0;
+        for (double x : e) t += x;
+        *energy = t;
+    }
+    if (forces) HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return CSG_OK;
+}
+
+csg_status csg_system_move(csg_ctx* c, const double* disp, int32_t transportVelocity) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
+    if (!disp) return fail(c, CSG_ERR_ARGUMENT, "null displacements");
+    HIPCHECK(c, hipSetDevice(c->device));
+    HIPCHECK(c, hipMemcpyAsync(S->disp.p, disp, 24 * (size_t)S->N, hipMemcpyHostToDevice, c->stream));
+    TransportParams T{};
+    T.mesh = c->dm;
+    T.n = S->N;
+    T.face = S->face.as<int32_t>();
+    T.bary = S->bary.as<double>();
+    T.disp = S->disp.as<double>();
+    T.scale = 1.0;
+    T.nVec = transportVelocity ? 1 : 0;
+    T.vecs = S->vel.as<double>();
+    T.status = S->tstatus.as<int32_t>();
+    T.maxCrossings = 100000;
+    HIPCHECK(c, launchTransport(T, c->stream));
+    hipLaunchKernelGGL(kCountNonzero, dim3(blocksFor(S->N)), dim3(kBlock), 0, c->stream, S->N, S->tstatus.as<int>(), S->counters.as<int>(),
+                       (int)CNT_WALK_FAIL, true);
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return CSG_OK;
+}
+
+csg_status csg_system_nve(csg_ctx* c, double dt, int32_t nSteps) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S || !S->configured) return fail(c, CSG_ERR_STATE, "csg_system_init and csg_system_configure first");
+    HIPCHECK(c, hipSetDevice(c->device));
+    const int N = S->N;
+    for (int step = 0; step < nSteps; ++step) {
+        hipLaunchKernelGGL(kVVFirst, dim3(blocksFor(N)), dim3(kBlock), 0, c->stream, N, dt, S->vel.as<double>(), S->force.as<double>(),
+                           S->disp.as<double>());
+        csg_status s = sysMove(c, S);
+        if (s) return s;
+        s = sysForces(c, S);
+        if (s) return s;
+        hipLaunchKernelGGL(kVVKick, dim3(blocksFor(N)), dim3(kBlock), 0, c->stream, N, dt, S->vel.as<double>(), S->force.as<double>());
+        HIPCHECK(c, hipGetLastError());
+    }
+    return CSG_OK;
+}
+
+csg_status csg_system_nvt(csg_ctx* c, double dt, double T, double tau, int32_t M, int32_t nSteps) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S || !S->configured) return fail(c, CSG_ERR_STATE, "csg_system_init and csg_system_configure first");
+    if (M < 1) return fail(c, CSG_ERR_ARGUMENT, "chain length must be at least 1");
+    HIPCHECK(c, hipSetDevice(c->device));
+    const int N = S->N;
+    if (!S->chainReady || S->M != M || S->T != T || S->tau != tau) {
+        // noseHooverNVT::setBathVariables (noseHooverNVT.cpp:28-36): Q0 = 2 (Ndof - 1) T tau^2, Qi = T tau^2, KE = Q0
+        std::vector<double> chain(4 * (size_t)(M + 1), 0.0);
+        chain[3] = 2.0 * (N - 1) * T * tau * tau;
+        for (int i = 1; i <= M; ++i) chain[4 * i + 3] = T * tau * tau;
+        double ke[2] = {chain[3], 1.0};
+        HIPCHECK(c, S->chain.ensure(8 * chain.size()));
+        HIPCHECK(c, hipMemcpyAsync(S->chain.p, chain.data(), 8 * chain.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(c, hipMemcpyAsync(S->ke.p, ke, 16, hipMemcpyHostToDevice, c->stream));
+        S->M = M;
+        S->T = T;
+        S->tau = tau;
+        S->chainReady = true;
+    }
+    const double dt2 = 0.5 * dt;
+    const int nb = blocksFor(N);
+    for (int step = 0; step < nSteps; ++step) {
+        hipLaunchKernelGGL(kNHChain, dim3(1), dim3(1), 0, c->stream, S->chain.as<double>(), S->ke.as<double>(), M, T, dt);
+        hipLaunchKernelGGL(kScaleVel, dim3(blocksFor(3 * N)), dim3(kBlock), 0, c->stream, N, S->vel.as<double>(), S->ke.as<double>());
+        hipLaunchKernelGGL(kScaledCopy, dim3(blocksFor(3 * N)), dim3(kBlock), 0, c->stream, N, dt2, S->vel.as<double>(), S->disp.as<double>());
+        csg_status s = sysMove(c, S);
+        if (s) return s;
+        s = sysForces(c, S);
+        if (s) return s;
+        hipLaunchKernelGGL(kNHKick, dim3(nb), dim3(kBlock), 0, c->stream, N, dt, dt2, S->vel.as<double>(), S->force.as<double>(),
+                           S->mass.as<double>(), S->disp.as<double>(), S->partial.as<double>());
+        hipLaunchKernelGGL(kSumPartials, dim3(1), dim3(1), 0, c->stream, nb, S->partial.as<double>(), S->ke.as<double>());
+        s = sysMove(c, S);
+        if (s) return s;
+        hipLaunchKernelGGL(kNHChain, dim3(1), dim3(1), 0, c->stream, S->chain.as<double>(), S->ke.as<double>(), M, T, dt);
+        hipLaunchKernelGGL(kScaleVel, dim3(blocksFor(3 * N)), dim3(kBlock), 0, c->stream, N, S->vel.as<double>(), S->ke.as<double>());
+        HIPCHECK(c, hipGetLastError());
+    }
+    return CSG_OK;
+}
+
+csg_status csg_system_sync(csg_ctx* c) {
+    if (!c) return CSG_ERR_ARGUMENT;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return CSG_OK;
+}
+
+csg_status csg_system_get(csg_ctx* c, int32_t* face, double* bary, double* vel, double* forces) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
+    HIPCHECK(c, hipSetDevice(c->device));
+    if (face) HIPCHECK(c, hipMemcpyAsync(face, S->face.p, 4 * (size_t)S->N, hipMemcpyDeviceToHost, c->stream));
+    if (bary) HIPCHECK(c, hipMemcpyAsync(bary, S->bary.p, 24 * (size_t)S->N, hipMemcpyDeviceToHost, c->stream));
+    if (vel) HIPCHECK(c, hipMemcpyAsync(vel, S->vel.p, 24 * (size_t)S->N, hipMemcpyDeviceToHost, c->stream));
+    if (forces) HIPCHECK(c, hipMemcpyAsync(forces, S->force.p, 24 * (size_t)S->N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return CSG_OK;
+}
+
+csg_status csg_system_status(csg_ctx* c, int64_t* pairsLastForce, int32_t* candOverflow, int32_t* unsolved, int32_t* walkFailures,
+                             double* kineticEnergy) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
+    HIPCHECK(c, hipSetDevice(c->device));
+    int cnt[CNT_N];
+    double ke[2] = {0, 0};
+    HIPCHECK(c, hipMemcpyAsync(cnt, S->counters.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    if (S->chainReady) HIPCHECK(c, hipMemcpyAsync(ke, S->ke.p, 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (pairsLastForce) *pairsLastForce = cnt[CNT_PAIRS];
+    if (candOverflow) *candOverflow = cnt[CNT_CAND_OVERFLOW];
+    if (unsolved) *unsolved = cnt[CNT_UNSOLVED];
+    if (walkFailures) *walkFailures = cnt[CNT_WALK_FAIL];
+    if (kineticEnergy) *kineticEnergy = ke[0];
+    return CSG_OK;
+}
+
+csg_status csg_system_profile(csg_ctx* c, int32_t enable) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
+    S->profiling = enable != 0;
+    return CSG_OK;
+}
+
+csg_status csg_system_kernel_times(csg_ctx* c, double* ms, int32_t* launches) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < MARK_N; ++k) {
+        if (ms) ms[k] = 0;
+        if (launches) launches[k] = 0;
+    }
+    for (auto& m : S->marks) {
+        float t = 0;
+        HIPCHECK(c, hipEventElapsedTime(&t, m.a, m.b));
+        if (ms) ms[m.kind] += t;
+        if (launches) launches[m.kind] += 1;
+        hipEventDestroy(m.a);
+        hipEventDestroy(m.b);
+    }
+    S->marks.clear();
+    return CSG_OK;
+}
+
+csg_status csg_system_patch_stats(csg_ctx* c, int64_t* sumFaces, int64_t* sumVerts, int32_t* maxFaces) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
+    std::vector<int> st(2 * (size_t)S->N);
+    HIPCHECK(c, hipMemcpy(st.data(), S->pstat.p, 8 * (size_t)S->N, hipMemcpyDeviceToHost));
+    int64_t f = 0, v = 0;
+    int mf = 0;
+    for (int i = 0; i < S->N; ++i) {
+        f += st[2 * i];
+        v += st[2 * i + 1];
+        mf = std::max(mf, st[2 * i]);
+    }
+    if (sumFaces) *sumFaces = f;
+    if (sumVerts) *sumVerts = v;
+    if (maxFaces) *maxFaces = mf;
+    return CSG_OK;
+}
+
+csg_status csg_system_fallback_counts(csg_ctx* c, int32_t* reasons) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
+    int cnt[CNT_N];
+    HIPCHECK(c, hipMemcpy(cnt, S->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
+    for (int b = 0; b < 6; ++b) reasons[b] = cnt[CNT_LDS_REASON + b];
+    return CSG_OK;
+}
+
+csg_status csg_system_pairs(csg_ctx* c, int32_t* offsets, int32_t* idx, int64_t cap) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
+    std::vector<int> off(S->N + 1);
+    HIPCHECK(c, hipMemcpy(off.data(), S->toff.p, 4 * (size_t)(S->N + 1), hipMemcpyDeviceToHost));
+    if (offsets) std::copy(off.begin(), off.end(), offsets);
+    if (idx) {
+        if (off[S->N] > cap) return fail(c, CSG_ERR_CAPACITY, "index buffer too small");
+        HIPCHECK(c, hipMemcpy(idx, S->cand.p, 4 * (size_t)off[S->N], hipMemcpyDeviceToHost));
+    }
+    return CSG_OK;
+}
+
+}  // extern "C"

@@ -5,13 +5,17 @@
 
 namespace csg {
 
-// LDS capacities of the fast variant: face hash slots (at most half filled), vertex hash slots (at most 3/4
-// filled), live window FIFO entries, targets per source. A source that exceeds any of them is solved again by the
-// scratch variant.
+// LDS capacities of the fast variant (one source per workgroup of kLdsThreads lanes, ~150 KB of the CU's 160 KB):
+// face and vertex hash slots (at most 3/4 filled), live work items of the FIFO, targets per source. A source that
+// exceeds any of them is solved again by the scratch variant.
 constexpr int kLdsFaceCap = 1024;
-constexpr int kLdsVertCap = 512;
-constexpr int kLdsQueueCap = 256;
+constexpr int kLdsVertCap = 1024;
+constexpr int kLdsQueueCap = 2048;
 constexpr int kLdsTargetCap = 256;
+constexpr int kLdsThreads = 256;
+
+// reasons a source exceeded the capacities of a kernel variant (bits of the per-source status)
+enum OverflowReason : int { OVF_FACES = 1, OVF_VERTS = 2, OVF_QUEUE = 4, OVF_TARGETS = 8, OVF_STEPS = 16, OVF_FRONTIER = 32 };
 
 struct GeoParams {
     DeviceMesh mesh;
@@ -30,7 +34,8 @@ struct GeoParams {
     uint32_t* pairFlags;
     double* forces;     // 3 per source
     double* energy;     // per source
-    int* status;        // per source: 0 solved, 1 capacity exceeded
+    int* status;        // per source: 0 solved, else OverflowReason bits
+    int* patchStats;    // per source: patch faces, patch vertices (optional)
     Potential pot;
     int useSentinel;    // unreachable pairs get (sentinelDist, (0,0,1)) as the reference does
     double sentinelDist;
@@ -46,5 +51,7 @@ size_t scratchBytesPerBlock(int fcap, int vcap, int qcap, int tcap);
 hipError_t launchGeodesicLds(const GeoParams& P, hipStream_t stream);
 hipError_t launchGeodesicScratch(const GeoParams& P, const ScratchLayout& L, const int32_t* list, int nList, int blocks,
                                  hipStream_t stream);
+// scratch variant over every source whose P.status is non-zero (left unsolved by the LDS variant); no host round trip
+hipError_t launchGeodesicScratchRedo(const GeoParams& P, const ScratchLayout& L, int blocks, hipStream_t stream);
 
 }  // namespace csg

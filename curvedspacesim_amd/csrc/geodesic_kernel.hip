@@ -33,6 +33,8 @@ constexpr double kInf = __builtin_huge_val();
 constexpr double kFilterTol = 1e-12;  // relative slack before a window is declared useless
 constexpr double kEdgeTol = 1e-12;    // relative distance at which an interval end counts as the edge's endpoint
 constexpr int kMaxSteps = 1 << 18;    // FIFO steps per source before it is abandoned as over capacity
+constexpr int kMaxWaves = 16;         // wavefronts per workgroup
+constexpr int kVertexFace = 1 << 30;  // tag of the work item that opens a pseudo-source window across one face
 
 __device__ inline unsigned long long dbits(double d) { return (unsigned long long)__double_as_longlong(d); }
 __device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
@@ -49,10 +51,11 @@ struct Tables {
     int32_t* vkey;   // vertex hash: global vertex id or -1
     unsigned long long* vdist;
     float2* vtan;    // start tangent of the best path to the vertex (source-face frame)
-    int32_t* vflg;   // 1: may act as a pseudo-source
+    uint8_t* vflg;   // 1: may act as a pseudo-source
     // FIFO of work items: windows and vertex events
     double *qb0, *qb1, *qsx, *qsy, *qsg;
-    int32_t* qfk;    // window: face slot * 8 + k * 2 + rooted; vertex event: -(vertex slot) - 1
+    int32_t* qfk;    // window: face slot * 8 + k * 2 + rooted (< 2^30); vertex event: -(vertex slot) - 1;
+                     // vertex-face item: kVertexFace | (face slot * 4 + corner of the vertex)
     float2* qt;      // rooted window: rotation (cos, sin) to the source-face frame; otherwise its fixed start tangent
     // targets of the source
     IT* tnext;
@@ -66,6 +69,7 @@ struct Tables {
 struct Shared {
     int head, tail, nFaces, nVerts, overflow, nCur, nNext, goals;
     double S[3], t1[3], t2[3];
+    double red[kMaxWaves][4];
 };
 
 template <class TT>
@@ -91,7 +95,7 @@ __device__ int faceInsert(const TT& T, Shared& sh, int g, bool& fresh) {
         if (k == -1) {
             if (atomicAdd(&sh.nFaces, 1) >= T.flimit) {
                 atomicAdd(&sh.nFaces, -1);
-                sh.overflow = 1;
+                atomicOr(&sh.overflow, OVF_FACES);
                 return -1;
             }
             int old = atomicCAS(&T.fkey[h], -1, g);
@@ -104,7 +108,7 @@ __device__ int faceInsert(const TT& T, Shared& sh, int g, bool& fresh) {
         }
         h = (h + 1) & T.fmask;
     }
-    sh.overflow = 1;
+    atomicOr(&sh.overflow, OVF_FACES);
     return -1;
 }
 
@@ -117,7 +121,7 @@ __device__ int vertInsert(const TT& T, Shared& sh, int v) {
         if (k == -1) {
             if (atomicAdd(&sh.nVerts, 1) >= T.vlimit) {
                 atomicAdd(&sh.nVerts, -1);
-                sh.overflow = 1;
+                atomicOr(&sh.overflow, OVF_VERTS);
                 return -1;
             }
             int old = atomicCAS(&T.vkey[h], -1, v);
@@ -127,7 +131,7 @@ __device__ int vertInsert(const TT& T, Shared& sh, int v) {
         }
         h = (h + 1) & T.vmask;
     }
-    sh.overflow = 1;
+    atomicOr(&sh.overflow, OVF_VERTS);
     return -1;
 }
 
@@ -216,7 +220,7 @@ __device__ void addWindow(const Ctx<TT>& C, int fs, int k, double L, double b0, 
     if (filtered(T, vA, vB, L, b0, b1, sx, sy, sg)) return;
     int pos = atomicAdd(&C.sh.tail, 1);
     if (pos - C.qBase >= T.qcap) {
-        C.sh.overflow = 1;
+        atomicOr(&C.sh.overflow, OVF_QUEUE);
         return;
     }
     int q = pos & (T.qcap - 1);
@@ -233,12 +237,25 @@ template <class TT>
 __device__ inline void pushVertexEvent(const Ctx<TT>& C, int vslot, double d) {
     int pos = atomicAdd(&C.sh.tail, 1);
     if (pos - C.qBase >= C.T.qcap) {
-        C.sh.overflow = 1;
+        atomicOr(&C.sh.overflow, OVF_QUEUE);
         return;
     }
     int q = pos & (C.T.qcap - 1);
     C.T.qsg[q] = d;
     C.T.qfk[q] = -vslot - 1;
+}
+
+template <class TT>
+__device__ inline void pushVertexFace(const Ctx<TT>& C, int gs, int j, double d, float2 tv) {
+    int pos = atomicAdd(&C.sh.tail, 1);
+    if (pos - C.qBase >= C.T.qcap) {
+        atomicOr(&C.sh.overflow, OVF_QUEUE);
+        return;
+    }
+    int q = pos & (C.T.qcap - 1);
+    C.T.qsg[q] = d;
+    C.T.qt[q] = tv;
+    C.T.qfk[q] = kVertexFace | (gs * 4 + j);
 }
 
 // pseudo-source point Q (global coordinates) in the frame of the window entering face g through its edge k
@@ -278,6 +295,7 @@ __device__ void windowTargets(const Ctx<TT>& C, int fs, int g, int k, double L, 
             atomicMin(&T.tbest[t], dbits(d));
         } else if (T.tbest[t] == dbits(d)) {
             T.tst[t] = startTangent(rooted, rot, rx, ry);
+            if (!P.pairEnd) continue;
             const DeviceMesh& M = P.mesh;
             double o[3], e[3];
             loadV(M, M.fv[3 * g + k], o);
@@ -293,15 +311,11 @@ __device__ void windowTargets(const Ctx<TT>& C, int fs, int g, int k, double L, 
 
 // Window propagation across the face it enters (pass 1): candidate for the opposite corner, up to two children.
 template <class TT>
-__device__ void propagate(const Ctx<TT>& C, int fk, double b0, double b1, double sx, double sy, double sg, float2 rot, VCand& cP0,
-                          VCand& cP1, VCand& cP2) {
+__device__ void propagate(const Ctx<TT>& C, int fk, int g, double L, double x2, double y2, double b0, double b1, double sx, double sy,
+                          double sg, float2 rot, VCand& cP0, VCand& cP1, VCand& cP2) {
     const TT& T = C.T;
-    const DeviceMesh& M = C.P.mesh;
     int fs = fk >> 3, k = (fk >> 1) & 3;
     bool rooted = fk & 1;
-    int g = T.fkey[fs];
-    const double* geo = M.fgeo + 9 * g + 3 * k;
-    double L = geo[0], x2 = geo[1], y2 = geo[2];
     int vC = T.fcv[3 * fs + (k + 2) % 3];
     windowTargets<false, TT>(C, fs, g, k, L, x2, y2, b0, b1, sx, sy, sg, rooted, rot);
     double xs = sx + (x2 - sx) * (-sy) / (y2 - sy);  // where the ray through the opposite corner crosses the entry edge
@@ -348,44 +362,19 @@ __device__ void propagate(const Ctx<TT>& C, int fk, double b0, double b1, double
     }
 }
 
-// Expansion of a pseudo-source vertex: a window on the far edge of every incident patch face. Pass 1 appends the
-// windows and records in `improved` (two bits per incident face) which endpoint distances it lowered; pass 2 gives
-// those endpoints the vertex's tangent and queues events for the ones that are pseudo-sources themselves.
+// A pseudo-source vertex event fans out into one work item per incident patch face; each item opens the window on the
+// face's far edge. The vertex's unfolded position in that window's frame comes from the face's precomputed frame of
+// that edge (the corner opposite it), so the item needs a single geometry load.
 template <class TT>
-__device__ void expandVertex(const Ctx<TT>& C, int vslot, double d, bool pass2, unsigned long long& improved) {
+__device__ void fanOutVertex(const Ctx<TT>& C, int vslot, double d) {
     const TT& T = C.T;
     const DeviceMesh& M = C.P.mesh;
     int v = T.vkey[vslot];
-    double Q[3];
-    loadV(M, v, Q);
     float2 tv = T.vtan[vslot];
-    const int q0 = M.vfoff[v], q1 = min(M.vfoff[v + 1], q0 + 32);
-    for (int q = q0; q < q1; ++q) {
-        int g = M.vflist[q] >> 2, j = M.vflist[q] & 3;
-        int gs = faceLookup(T, g);
-        if (gs < 0) continue;
-        int nb = T.fnb[3 * gs + (j + 1) % 3];
-        if (nb < 0) continue;
-        int hs = nb >> 2, kh = nb & 3;
-        double sx, sy, L;
-        pointInFrame(M, T.fkey[hs], kh, Q, sx, sy, L);
-        const int bit = 2 * (q - q0);
-        if (!pass2) {
-            VCand a = noCand(), b = noCand();
-            addWindow(C, hs, kh, L, 0.0, L, sx, sy, d, false, tv, a, b);
-            if (a.improved) improved |= 1ull << bit;
-            if (b.improved) improved |= 2ull << bit;
-        } else {
-            int ends[2] = {T.fcv[3 * hs + kh], T.fcv[3 * hs + (kh + 1) % 3]};
-            double dd[2] = {d + sqrt(sx * sx + sy * sy), d + sqrt((L - sx) * (L - sx) + sy * sy)};
-            for (int e = 0; e < 2; ++e) {
-                if (!((improved >> (bit + e)) & 1)) continue;
-                int u = ends[e];
-                if (T.vdist[u] != dbits(dd[e])) continue;
-                T.vtan[u] = tv;
-                if (T.vflg[u] && dd[e] <= C.rMax) pushVertexEvent(C, u, dd[e]);
-            }
-        }
+    for (int q = M.vfoff[v]; q < M.vfoff[v + 1]; ++q) {
+        int e = M.vflist[q];
+        int gs = faceLookup(T, e >> 2);
+        if (gs >= 0 && T.fnb[3 * gs + ((e & 3) + 1) % 3] >= 0) pushVertexFace(C, gs, e & 3, d, tv);
     }
 }
 
@@ -411,7 +400,7 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int src,
     int* nxt = (int*)T.qb1;
     const int frontCap = 2 * T.qcap;
     if (P.patchRadius == nullptr) {
-        for (int g = lane; g < M.nF; g += 64) {
+        for (int g = lane; g < M.nF; g += blockDim.x) {
             bool fresh;
             faceInsert(T, sh, g, fresh);
         }
@@ -421,7 +410,7 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int src,
         int nbr[3];
         for (int k = 0; k < 3; ++k) nbr[k] = M.fadj[3 * sf + k] < 0 ? -2 : (M.fadj[3 * sf + k] >> 2);
         int anyGoal = 0, beyond = 0;
-        for (int t = lane; t < nT; t += 64) {
+        for (int t = lane; t < nT; t += blockDim.x) {
             int f = P.pface[P.tidx[tb + t]];
             if (f != sf) anyGoal = 1;
             if (f != sf && f != nbr[0] && f != nbr[1] && f != nbr[2]) beyond = 1;
@@ -445,7 +434,7 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int src,
             while (true) {
                 const int nc = sh.nCur;
                 if (nc == 0 || sh.overflow) break;
-                for (int i = lane; i < nc; i += 64) {
+                for (int i = lane; i < nc; i += blockDim.x) {
                     int f = cur[i];
                     for (int k = 0; k < 3; ++k) {
                         int a = M.fadj[3 * f + k];
@@ -467,13 +456,13 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int src,
                             if (pos < frontCap)
                                 nxt[pos] = g;
                             else
-                                sh.overflow = 1;
+                                atomicOr(&sh.overflow, OVF_FRONTIER);
                         }
                     }
                 }
                 __syncthreads();
                 const int nn = min(sh.nNext, frontCap);
-                for (int i = lane; i < nn; i += 64) cur[i] = nxt[i];
+                for (int i = lane; i < nn; i += blockDim.x) cur[i] = nxt[i];
                 __syncthreads();
                 if (lane == 0) {
                     sh.nCur = nn;
@@ -483,14 +472,14 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int src,
             }
         }
         // target faces not reached above (possibly disconnected from the source)
-        for (int t = lane; t < nT; t += 64) {
+        for (int t = lane; t < nT; t += blockDim.x) {
             bool fresh;
             faceInsert(T, sh, P.pface[P.tidx[tb + t]], fresh);
         }
     }
     __syncthreads();
     if (sh.overflow) return false;
-    for (int s = lane; s <= T.fmask; s += 64) {
+    for (int s = lane; s <= T.fmask; s += blockDim.x) {
         int g = T.fkey[s];
         if (g < 0) continue;
         for (int k = 0; k < 3; ++k) {
@@ -502,7 +491,7 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int src,
     }
     __syncthreads();
     if (sh.overflow) return false;
-    for (int s = lane; s <= T.vmask; s += 64) {
+    for (int s = lane; s <= T.vmask; s += blockDim.x) {
         int v = T.vkey[s];
         if (v < 0) continue;
         int fl = (M.vflag[v] & 3) ? 1 : 0;
@@ -517,7 +506,7 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int src,
         T.vtan[s] = make_float2(0.f, 0.f);
     }
     // per-face target lists: slots looked up in parallel (parked in tbest), linked by one lane
-    for (int t = lane; t < nT; t += 64) T.tbest[t] = (unsigned long long)faceLookup(T, P.pface[P.tidx[tb + t]]);
+    for (int t = lane; t < nT; t += blockDim.x) T.tbest[t] = (unsigned long long)faceLookup(T, P.pface[P.tidx[tb + t]]);
     __syncthreads();
     if (lane == 0)
         for (int t = nT - 1; t >= 0; --t) {
@@ -526,7 +515,7 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int src,
             T.fhead[s] = t;
         }
     __syncthreads();
-    for (int t = lane; t < nT; t += 64) T.tbest[t] = dbits(kInf);
+    for (int t = lane; t < nT; t += blockDim.x) T.tbest[t] = dbits(kInf);
     __syncthreads();
     return true;
 }
@@ -551,9 +540,6 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
     }
     __syncthreads();
     if (lane < 3) {
-        int vs = T.fcv[3 * ss + lane];
-        double d = bitsd(T.vdist[vs]);
-        if (T.vflg[vs] && d <= C.rMax) pushVertexEvent(C, vs, d);
         int nb = T.fnb[3 * ss + lane];
         if (nb >= 0) {
             int hs = nb >> 2, kh = nb & 3, h = T.fkey[hs];
@@ -570,67 +556,84 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
         }
     }
     __syncthreads();
+    // the windows above may lower a corner's distance by rounding (it is recomputed in their edge frame), so the
+    // pseudo-source events of the corners carry the final values; an event with a superseded value would be skipped
+    if (lane < 3) {
+        int vs = T.fcv[3 * ss + lane];
+        double d = bitsd(T.vdist[vs]);
+        if (T.vflg[vs] && d <= C.rMax) pushVertexEvent(C, vs, d);
+    }
+    __syncthreads();
     for (int step = 0;; ++step) {
         const int h = sh.head, t = sh.tail;
         if (h >= t || sh.overflow) break;
         if (step >= kMaxSteps) {  // every source ends: an unfinished one is reported like a capacity overflow
-            if (lane == 0) sh.overflow = 1;
+            if (lane == 0) atomicOr(&sh.overflow, OVF_STEPS);
             break;
         }
-        const int nb = min(64, t - h);
+        const int nb = min((int)blockDim.x, t - h);
         C.qBase = h;
         const bool mine = lane < nb;
         const int q = (h + lane) & (T.qcap - 1);
-        int fk = 0;
-        double b0 = 0, b1 = 0, sx = 0, sy = 0, sg = 0;
+        int fk = -1, kind = 0, g = -1, fs = 0, k = 0;  // kind: 0 window, 1 vertex event, 2 vertex-face item
+        double b0 = 0, b1 = 0, sx = 0, sy = 0, sg = 0, L = 0, x2 = 0, y2 = 0;
         float2 rot = make_float2(0.f, 0.f);
         if (mine) {
             fk = T.qfk[q];
             sg = T.qsg[q];
-            if (fk >= 0) {
+            rot = T.qt[q];
+            if (fk >= kVertexFace) {
+                kind = 2;
+                fs = (fk & (kVertexFace - 1)) >> 2;
+                k = (fk & 3) + 1;  // the far edge of the face, seen from the vertex at corner (fk & 3)
+                if (k == 3) k = 0;
+            } else if (fk >= 0) {
+                kind = 0;
+                fs = fk >> 3;
+                k = (fk >> 1) & 3;
                 b0 = T.qb0[q];
                 b1 = T.qb1[q];
                 sx = T.qsx[q];
                 sy = T.qsy[q];
-                rot = T.qt[q];
-            }
-        }
-        bool live = false;
-        if (mine) {
-            if (fk >= 0) {
-                int fs = fk >> 3, k = (fk >> 1) & 3;
-                double L = M.fgeo[9 * T.fkey[fs] + 3 * k];
-                live = keyOf(b0, b1, sx, sy, sg) <= C.rMax &&
-                       !filtered(T, T.fcv[3 * fs + k], T.fcv[3 * fs + (k + 1) % 3], L, b0, b1, sx, sy, sg);
             } else {
-                live = T.vdist[-fk - 1] == dbits(sg) && sg <= C.rMax;
+                kind = 1;
+            }
+            if (kind != 1) {  // geometry of the edge the item works on, fetched before the barrier
+                g = T.fkey[fs];
+                const double* geo = M.fgeo + 9 * g + 3 * k;
+                L = geo[0];
+                x2 = geo[1];
+                y2 = geo[2];
             }
         }
         __syncthreads();
         VCand cP0 = noCand(), cP1 = noCand(), cP2 = noCand();
-        unsigned long long improved = 0;
-        if (live) {
-            if (fk >= 0)
-                propagate(C, fk, b0, b1, sx, sy, sg, rot, cP0, cP1, cP2);
-            else
-                expandVertex(C, -fk - 1, sg, false, improved);
+        bool live = false;
+        if (mine) {
+            if (kind == 0) {
+                live = keyOf(b0, b1, sx, sy, sg) <= C.rMax &&
+                       !filtered(T, T.fcv[3 * fs + k], T.fcv[3 * fs + (k + 1) % 3], L, b0, b1, sx, sy, sg);
+                if (live) propagate(C, fk, g, L, x2, y2, b0, b1, sx, sy, sg, rot, cP0, cP1, cP2);
+            } else if (kind == 1) {
+                int vs = -fk - 1;
+                if (T.vdist[vs] == dbits(sg) && sg <= C.rMax) fanOutVertex(C, vs, sg);
+            } else {
+                int vs = T.fcv[3 * fs + (fk & 3)];
+                live = T.vdist[vs] == dbits(sg);  // a later, shorter path to the vertex supersedes this item
+                if (live) {
+                    int nbk = T.fnb[3 * fs + k];
+                    // the vertex unfolded into the neighbour's frame of the same edge (origin at the edge's far end)
+                    addWindow(C, nbk >> 2, nbk & 3, L, 0.0, L, L - x2, -y2, sg, false, rot, cP0, cP1);
+                }
+            }
         }
         __syncthreads();
         if (live) {
-            if (fk >= 0) {
-                int fs = fk >> 3;
-                if (T.fhead[fs] >= 0) {
-                    int k = (fk >> 1) & 3;
-                    int g = T.fkey[fs];
-                    const double* geo = M.fgeo + 9 * g + 3 * k;
-                    windowTargets<true, TT>(C, fs, g, k, geo[0], geo[1], geo[2], b0, b1, sx, sy, sg, fk & 1, rot);
-                }
-                finishVertexCand(C, cP0);
-                finishVertexCand(C, cP1);
-                finishVertexCand(C, cP2);
-            } else {
-                expandVertex(C, -fk - 1, sg, true, improved);
-            }
+            if (kind == 0 && T.fhead[fs] >= 0)
+                windowTargets<true, TT>(C, fs, g, k, L, x2, y2, b0, b1, sx, sy, sg, fk & 1, rot);
+            finishVertexCand(C, cP0);
+            finishVertexCand(C, cP1);
+            finishVertexCand(C, cP2);
         }
         __syncthreads();
         if (lane == 0) sh.head = h + nb;
@@ -645,15 +648,15 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
     const int pi = P.srcIdx ? P.srcIdx[src] : src;
     const int sf = P.pface[pi];
     const int tb = P.toff[src], nT = P.toff[src + 1] - tb;
-    for (int i = lane; i <= T.fmask; i += 64) {
+    for (int i = lane; i <= T.fmask; i += blockDim.x) {
         T.fkey[i] = -1;
         T.fhead[i] = -1;
     }
-    for (int i = lane; i <= T.vmask; i += 64) T.vkey[i] = -1;
-    for (int i = lane; i < min(nT, T.tcap); i += 64) T.tbest[i] = dbits(kInf);
+    for (int i = lane; i <= T.vmask; i += blockDim.x) T.vkey[i] = -1;
+    for (int i = lane; i < min(nT, T.tcap); i += blockDim.x) T.tbest[i] = dbits(kInf);
     if (lane == 0) {
         sh.head = sh.tail = sh.nFaces = sh.nVerts = 0;
-        sh.overflow = nT > T.tcap ? 1 : 0;
+        sh.overflow = nT > T.tcap ? OVF_TARGETS : 0;
         sh.nCur = sh.nNext = sh.goals = 0;
         const double* bb = P.pbary + 3 * pi;
         double s = bb[0] + bb[1] + bb[2];
@@ -679,7 +682,7 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
     // ---- finalize: paths through the target face's corners, direct visibility, outputs, force
     const bool ovf = sh.overflow != 0;
     double F0 = 0, F1 = 0, F2 = 0, E = 0;
-    for (int t = lane; t < nT; t += 64) {
+    for (int t = lane; t < nT; t += blockDim.x) {
         const int pj = P.tidx[tb + t];
         const int out = tb + t;
         if (ovf) {
@@ -769,6 +772,25 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
     F1 = wsum(F1);
     F2 = wsum(F2);
     E = wsum(E);
+    if (blockDim.x > 64) {  // combine the wavefronts' partial sums in fixed order
+        __syncthreads();
+        if ((lane & 63) == 0) {
+            sh.red[lane >> 6][0] = F0;
+            sh.red[lane >> 6][1] = F1;
+            sh.red[lane >> 6][2] = F2;
+            sh.red[lane >> 6][3] = E;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            F0 = F1 = F2 = E = 0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+                F0 += sh.red[w][0];
+                F1 += sh.red[w][1];
+                F2 += sh.red[w][2];
+                E += sh.red[w][3];
+            }
+        }
+    }
     if (lane == 0) {
         if (P.forces) {
             P.forces[3 * src] = F0;
@@ -776,7 +798,11 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
             P.forces[3 * src + 2] = F2;
         }
         if (P.energy) P.energy[src] = E;
-        if (P.status) P.status[src] = ovf ? 1 : 0;
+        if (P.status) P.status[src] = sh.overflow;
+        if (P.patchStats) {
+            P.patchStats[2 * src] = sh.nFaces;
+            P.patchStats[2 * src + 1] = sh.nVerts;
+        }
     }
     __syncthreads();
 }
@@ -784,11 +810,12 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
 }  // namespace
 
 // LDS variant: tables in __shared__ memory, one source per workgroup
-template <int FH, int VH, int QC, int TC>
-__global__ __launch_bounds__(64) void geodesicForceLds(GeoParams P) {
+template <int FH, int VH, int QC, int TC, int NT>
+__global__ __launch_bounds__(NT) void geodesicForceLds(GeoParams P) {
     __shared__ int32_t fkey[FH];
     __shared__ int16_t fhead[FH], fcv[3 * FH], fnb[3 * FH];
-    __shared__ int32_t vkey[VH], vflg[VH];
+    __shared__ int32_t vkey[VH];
+    __shared__ uint8_t vflg[VH];
     __shared__ unsigned long long vdist[VH];
     __shared__ float2 vtan[VH];
     __shared__ double qb0[QC], qb1[QC], qsx[QC], qsy[QC], qsg[QC];
@@ -801,8 +828,9 @@ __global__ __launch_bounds__(64) void geodesicForceLds(GeoParams P) {
     __shared__ Shared sh;
     static_assert(2 * QC >= FH / 2, "BFS frontiers live in the FIFO storage");
     static_assert(4 * FH <= 32767 && VH <= 32767 && TC <= 32767, "slots must fit int16");
+    static_assert(NT % 64 == 0 && NT / 64 <= kMaxWaves, "whole wavefronts");
     Tables<int16_t> T{fkey, fcv, fnb, fhead, vkey, vdist, vtan, vflg, qb0, qb1, qsx, qsy, qsg, qfk, qt, tnext, tbest, tst, tet,
-             FH - 1, VH - 1, QC, TC, FH / 2, (VH * 3) / 4};
+             FH - 1, VH - 1, QC, TC, (FH * 3) / 4, (VH * 3) / 4};
     const int src = blockIdx.x;
     if (src >= P.nSrc) return;
     solveSource(P, T, sh, src);
@@ -810,7 +838,7 @@ __global__ __launch_bounds__(64) void geodesicForceLds(GeoParams P) {
 
 // Scratch variant: the same solver with its tables in a per-workgroup slice of an HBM buffer, persistent over a
 // list of sources (sources the LDS variant could not hold, or whole-mesh solves).
-__global__ __launch_bounds__(64) void geodesicForceScratch(GeoParams P, ScratchLayout L, const int32_t* list, int nList) {
+__global__ __launch_bounds__(64) void geodesicForceScratch(GeoParams P, ScratchLayout L, const int32_t* list, int nList, bool redoOnly) {
     __shared__ Shared sh;
     char* base = (char*)L.base + (size_t)blockIdx.x * L.bytesPerBlock;
     size_t off = 0;
@@ -825,7 +853,7 @@ __global__ __launch_bounds__(64) void geodesicForceScratch(GeoParams P, ScratchL
     T.fcv = (int32_t*)take(12 * (size_t)L.fcap);
     T.fnb = (int32_t*)take(12 * (size_t)L.fcap);
     T.vkey = (int32_t*)take(4 * (size_t)L.vcap);
-    T.vflg = (int32_t*)take(4 * (size_t)L.vcap);
+    T.vflg = (uint8_t*)take((size_t)L.vcap);
     T.vdist = (unsigned long long*)take(8 * (size_t)L.vcap);
     T.vtan = (float2*)take(8 * (size_t)L.vcap);
     T.qb0 = (double*)take(8 * (size_t)L.qcap);
@@ -845,7 +873,11 @@ __global__ __launch_bounds__(64) void geodesicForceScratch(GeoParams P, ScratchL
     T.tcap = L.tcap;
     T.flimit = L.fcap / 2;
     T.vlimit = (L.vcap * 3) / 4;
-    for (int i = blockIdx.x; i < nList; i += gridDim.x) solveSource(P, T, sh, list ? list[i] : i);
+    for (int i = blockIdx.x; i < nList; i += gridDim.x) {
+        const int src = list ? list[i] : i;
+        if (redoOnly && P.status[src] == 0) continue;  // solved by the LDS variant
+        solveSource(P, T, sh, src);
+    }
 }
 
 size_t scratchBytesPerBlock(int fcap, int vcap, int qcap, int tcap) {
@@ -856,14 +888,21 @@ size_t scratchBytesPerBlock(int fcap, int vcap, int qcap, int tcap) {
 
 hipError_t launchGeodesicLds(const GeoParams& P, hipStream_t stream) {
     if (P.nSrc <= 0) return hipSuccess;
-    hipLaunchKernelGGL((geodesicForceLds<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap>), dim3(P.nSrc), dim3(64), 0, stream, P);
+    hipLaunchKernelGGL((geodesicForceLds<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsThreads>), dim3(P.nSrc),
+                       dim3(kLdsThreads), 0, stream, P);
     return hipGetLastError();
 }
 
 hipError_t launchGeodesicScratch(const GeoParams& P, const ScratchLayout& L, const int32_t* list, int nList, int blocks,
                                  hipStream_t stream) {
     if (nList <= 0) return hipSuccess;
-    hipLaunchKernelGGL(geodesicForceScratch, dim3(blocks), dim3(64), 0, stream, P, L, list, nList);
+    hipLaunchKernelGGL(geodesicForceScratch, dim3(blocks), dim3(64), 0, stream, P, L, list, nList, false);
+    return hipGetLastError();
+}
+
+hipError_t launchGeodesicScratchRedo(const GeoParams& P, const ScratchLayout& L, int blocks, hipStream_t stream) {
+    if (P.nSrc <= 0) return hipSuccess;
+    hipLaunchKernelGGL(geodesicForceScratch, dim3(blocks), dim3(64), 0, stream, P, L, (const int32_t*)nullptr, P.nSrc, true);
     return hipGetLastError();
 }
 

@@ -98,6 +98,50 @@ csg_status csg_transport_batch(csg_ctx* ctx, int32_t n, int32_t* face, double* b
 /* --- meshPositionToEuclideanLocation (src/models/triangulatedMeshSpace.cpp:82-106) ------------------------------- */
 csg_status csg_positions(csg_ctx* ctx, int32_t n, const int32_t* face, const double* bary, double* xyz);
 
+/* --- device-resident particle system: the batched seam of simpleModel (src/models/simpleModel.h:25-111) ----------
+ * Positions, velocities, masses and forces live in HBM; every step is a fixed sequence of launches on the context
+ * stream with no host synchronisation. Errors inside a step (candidate capacity, an unsolved source, a failed walk)
+ * are counted on the device and reported by csg_system_status. */
+/* simpleModel::initializeSimpleModel / setParticlePositions (simpleModel.cpp:7-35, :318-326); vel and mass nullable
+ * (zero velocities, unit masses) */
+csg_status csg_system_init(csg_ctx* ctx, int32_t N, const int32_t* face, const double* bary, const double* vel, const double* mass);
+/* force computer and neighbour structure: pot and its range (force::maximumInteractionRange, baseForce.h:57); cellList
+ * 1 = cellListNeighborStructure (cellListNeighborStructure.cpp), 0 = all-to-all baseNeighborStructure; windows with a
+ * distance bound above rMax are not propagated (use the potential's cutoff, or a huge value); maxPatchFaces sizes the
+ * HBM tables for sources the LDS tables cannot hold (0: 1536 faces; at most 4096 targets per source with a cell
+ * list). Uses the context's submeshing setting. */
+csg_status csg_system_configure(csg_ctx* ctx, const csg_potential* pot, double range, int32_t cellList, double rMax, int32_t maxPatchFaces);
+/* simulation::computeForces (simulation.cpp:61-70): findNeighbors + pair forces; forces (3N) and energy nullable */
+csg_status csg_system_compute_forces(csg_ctx* ctx, double* forces, double* energy);
+/* simpleModel::moveParticles (simpleModel.cpp:44-66) with host displacements (3N); velocities transported if asked */
+csg_status csg_system_move(csg_ctx* ctx, const double* disp, int32_t transportVelocity);
+/* nSteps of velocityVerletNVE::performUpdate (velocityVerletNVE.cpp:3-29), asynchronous */
+csg_status csg_system_nve(csg_ctx* ctx, double dt, int32_t nSteps);
+/* nSteps of noseHooverNVT::performUpdate (noseHooverNVT.cpp:42-139) with chain length M, asynchronous; the chain is
+ * initialised on the first call (setBathVariables, :28-36) and kept across calls with the same (T, tau, M) */
+csg_status csg_system_nvt(csg_ctx* ctx, double dt, double T, double tau, int32_t M, int32_t nSteps);
+/* waits for the context stream */
+csg_status csg_system_sync(csg_ctx* ctx);
+/* copies state back (every pointer nullable): faces N, barycentric 3N, velocities 3N, forces 3N */
+csg_status csg_system_get(csg_ctx* ctx, int32_t* face, double* bary, double* vel, double* forces);
+/* counters since csg_system_init: pairs of the last force computation, candidate-capacity overflows, sources left
+ * unsolved, failed walks; and the thermostat's kinetic energy (0 before any NVT step) */
+csg_status csg_system_status(csg_ctx* ctx, int64_t* pairsLastForce, int32_t* candOverflow, int32_t* unsolved, int32_t* walkFailures,
+                             double* kineticEnergy);
+/* event timing of the step's launches (no reference counterpart; replaces the wall-clock profiler of
+ * src/utility/profiler.h:16-59): when enabled, HIP events on the context stream bracket each launch group; ms[4] and
+ * launches[4] receive the totals since the last query for {neighbour search, geodesic LDS kernel, geodesic HBM
+ * kernel, walk kernel} */
+csg_status csg_system_profile(csg_ctx* ctx, int32_t enable);
+csg_status csg_system_kernel_times(csg_ctx* ctx, double* ms, int32_t* launches);
+/* patch sizes of the last force computation: sums over sources of patch faces and vertices, largest patch */
+csg_status csg_system_patch_stats(csg_ctx* ctx, int64_t* sumFaces, int64_t* sumVerts, int32_t* maxFaces);
+/* sources handed from the LDS kernel to the HBM kernel since csg_system_init, by reason: reasons[6] = {patch faces,
+ * patch vertices, live windows, targets, step limit, search frontier} */
+csg_status csg_system_fallback_counts(csg_ctx* ctx, int32_t* reasons);
+/* candidate lists of the last force computation (offsets N+1, indices up to cap) */
+csg_status csg_system_pairs(csg_ctx* ctx, int32_t* offsets, int32_t* idx, int64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -228,3 +228,100 @@ def pair_force(kind, params, d):
     sc = C.c_double()
     e = L.orc_pair_force(int(kind), np.ascontiguousarray(params, np.float64), float(d), C.byref(sc))
     return e, sc.value
+
+
+class OracleSystem:
+    """Reference control flow of the integrators over the oracle's forces and walk (test infrastructure).
+
+    velocityVerletNVE::performUpdate (src/updaters/velocityVerletNVE.cpp:3-29) and noseHooverNVT::performUpdate with
+    propagateChain / propagatePositionsVelocities (src/updaters/noseHooverNVT.cpp:3-150), including the chain
+    initialisation of setBathVariables (kinetic energy set to Q0).
+    """
+
+    def __init__(self, mesh, face, bary, vel, potential, rng_range, cell_list, submeshing, maximum_distance,
+                 mass=None, threads=8):
+        self.m = mesh
+        self.face = np.array(face, np.int32).copy()
+        self.bary = np.array(bary, np.float64).reshape(-1, 3).copy()
+        self.vel = np.array(vel, np.float64).reshape(-1, 3).copy()
+        self.N = len(self.face)
+        self.F = np.zeros((self.N, 3))
+        self.mass = np.ones(self.N) if mass is None else np.asarray(mass, float)
+        self.kind = int(potential[0])
+        self.params = np.array(list(potential[1:]) + [0.0] * (3 - len(potential[1:])), float)
+        self.range = rng_range
+        self.cell_list = cell_list
+        self.submeshing = submeshing
+        self.maximum_distance = maximum_distance
+        self.threads = threads
+        X = mesh.xyz
+        self.lo, self.hi = np.minimum(X.min(0), 0), np.maximum(X.max(0), 0)
+        self.chain = None
+
+    def candidates(self):
+        P = self.m.positions(self.face, self.bary)
+        if self.cell_list:
+            return cell_candidates(P, self.lo, self.hi, self.range, self.range)
+        N = self.N
+        off = np.arange(0, N * (N - 1) + 1, N - 1, dtype=np.int32)
+        idx = np.array([j for i in range(N) for j in range(N) if j != i], np.int32)
+        return off, idx, np.full(N, 1e20)
+
+    def compute_forces(self):
+        off, idx, md = self.candidates()
+        r = self.m.compute_forces(self.face, self.bary, off, idx, md, self.kind, self.params, self.submeshing,
+                                  self.maximum_distance, threads=self.threads)
+        self.F = r["forces"]
+        self.last = dict(off=off, idx=idx, **r)
+        return r
+
+    def move(self, disp):
+        self.face, self.bary, self.vel, bad = self.m.move(self.face, self.bary, disp, self.vel, threads=self.threads)
+        if bad:
+            raise RuntimeError(f"{bad} walks failed")
+
+    def nve_step(self, dt):
+        disp = dt * self.vel + 0.5 * dt * dt * self.F
+        self.vel = self.vel + 0.5 * dt * self.F
+        self.move(disp)
+        self.compute_forces()
+        self.vel = self.vel + 0.5 * dt * self.F
+
+    def _chain_half(self):
+        b, M, T, dt = self.chain, self.M, self.T, self.dt
+        dt2, dt4, dt8 = 0.5 * dt, 0.25 * dt, 0.125 * dt
+        for ii in range(M - 1, 0, -1):
+            b[ii, 2] = (b[ii - 1, 3] * b[ii - 1, 1] ** 2 - T) / b[ii, 3]
+            ef = np.exp(-dt8 * b[ii + 1, 1])
+            b[ii, 1] = (b[ii, 1] * ef + b[ii, 2] * dt4) * ef
+        b[0, 2] = 2.0 * self.KE / b[0, 3] - 1.0
+        ef = np.exp(-dt8 * b[1, 1])
+        b[0, 1] = (b[0, 1] * ef + b[0, 2] * dt4) * ef
+        for ii in range(M):
+            b[ii, 0] += dt2 * b[ii, 1]
+        s = np.exp(-dt2 * b[0, 1])
+        self.KE = s * s * self.KE
+        b[0, 2] = 2.0 * self.KE / b[0, 3] - 1.0
+        ef = np.exp(-dt8 * b[1, 1])
+        b[0, 1] = (b[0, 1] * ef + b[0, 2] * dt4) * ef
+        for ii in range(1, M):
+            b[ii, 2] = (b[ii - 1, 3] * b[ii - 1, 1] ** 2 - T) / b[ii, 3]
+            ef = np.exp(-dt8 * b[ii + 1, 1])
+            b[ii, 1] = (b[ii, 1] * ef + b[ii, 2] * dt4) * ef
+        return s
+
+    def nvt_step(self, dt, T, tau=1.0, M=2):
+        if self.chain is None:
+            self.M, self.T, self.dt = M, T, dt
+            self.chain = np.zeros((M + 1, 4))
+            self.chain[0, 3] = 2.0 * (self.N - 1) * T * tau * tau
+            self.chain[1:, 3] = T * tau * tau
+            self.KE = self.chain[0, 3]
+        dt2 = 0.5 * dt
+        self.vel = self.vel * self._chain_half()
+        self.move(dt2 * self.vel)
+        self.compute_forces()
+        self.vel = self.vel + (dt / self.mass)[:, None] * self.F
+        self.KE = float((0.5 * self.mass * (self.vel * self.vel).sum(1)).sum())
+        self.move(dt2 * self.vel)
+        self.vel = self.vel * self._chain_half()
