@@ -135,6 +135,9 @@ def main():
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kt.items()},
         "mean_patch_faces": ps["faces"] / len(w.face),
         "max_patch_faces": ps["max_faces"],
+        "mean_fifo_steps": ps["steps"] / len(w.face),
+        "mean_work_items": ps["items"] / len(w.face),
+        "mean_source_us": {k: ps["us_" + k] / len(w.face) for k in ("patch", "propagate", "epilogue")},
         "lds_fallbacks_per_step": {k: v / (args.steps + args.warmup) for k, v in sysm.fallback_counts().items()},
         "roofline": {"kernel": "geodesicForceLds", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,

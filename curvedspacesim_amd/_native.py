@@ -71,7 +71,7 @@ def csg():
     L.csg_system_pairs.argtypes = [VP, VP, VP, C.c_int64]
     L.csg_system_profile.argtypes = [VP, C.c_int32]
     L.csg_system_kernel_times.argtypes = [VP, _f64p, _i32p]
-    L.csg_system_patch_stats.argtypes = [VP, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+    L.csg_system_patch_stats.argtypes = [VP, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), _i32p]
     L.csg_system_fallback_counts.argtypes = [VP, _i32p]
     _csg = L
     return L
@@ -303,10 +303,10 @@ class System:
         self.ctx._check(self.L.csg_system_profile(self.ctx.h, int(bool(enable))), "csg_system_profile")
 
     def kernel_times(self):
-        ms = np.zeros(4)
-        n = np.zeros(4, np.int32)
+        ms = np.zeros(5)
+        n = np.zeros(5, np.int32)
         self.ctx._check(self.L.csg_system_kernel_times(self.ctx.h, ms, n), "csg_system_kernel_times")
-        names = ["neighbors", "geodesic_lds", "geodesic_hbm", "transport"]
+        names = ["neighbors", "geodesic_lds", "geodesic_hbm", "transport", "patch"]
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(names)}
 
     def fallback_counts(self):
@@ -315,7 +315,9 @@ class System:
         return dict(zip(["faces", "verts", "queue", "targets", "steps", "frontier"], r.tolist()))
 
     def patch_stats(self):
-        f, v, m = C.c_int64(), C.c_int64(), C.c_int32()
-        self.ctx._check(self.L.csg_system_patch_stats(self.ctx.h, C.byref(f), C.byref(v), C.byref(m)),
-                        "csg_system_patch_stats")
-        return dict(faces=f.value, verts=v.value, max_faces=m.value)
+        s = np.zeros(7, np.int64)
+        m = np.zeros(7, np.int32)
+        self.ctx._check(self.L.csg_system_patch_stats(self.ctx.h, s, m), "csg_system_patch_stats")
+        return dict(faces=int(s[0]), verts=int(s[1]), steps=int(s[2]), items=int(s[3]), max_faces=int(m[0]),
+                    max_steps=int(m[2]), max_items=int(m[3]), us_patch=s[4] * 0.01, us_propagate=s[5] * 0.01,
+                    us_epilogue=s[6] * 0.01)

@@ -148,6 +148,14 @@ csg_status runGeodesics(csg_ctx* c, csg::GeoParams& G, int maxTargets, bool whol
     HIPCHECK(c, c->status.ensure(sizeof(int) * (size_t)G.nSrc));
     G.status = c->status.as<int>();
     if (!wholeMesh) {
+        if (G.patchRadius) {
+            HIPCHECK(c, c->plist.ensure(4 * (size_t)csg::kPatchListCap * G.nSrc));
+            HIPCHECK(c, c->pcount.ensure(4 * (size_t)G.nSrc));
+            G.patchList = c->plist.as<int32_t>();
+            G.patchCount = c->pcount.as<int32_t>();
+            G.patchStride = csg::kPatchListCap;
+            HIPCHECK(c, csg::launchPatch(G, c->plist.as<int32_t>(), c->pcount.as<int32_t>(), G.patchStride, c->stream));
+        }
         HIPCHECK(c, csg::launchGeodesicLds(G, c->stream));
         std::vector<int> st(G.nSrc);
         HIPCHECK(c, hipMemcpyAsync(st.data(), G.status, sizeof(int) * G.nSrc, hipMemcpyDeviceToHost, c->stream));

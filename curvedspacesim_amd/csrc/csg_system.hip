@@ -35,7 +35,7 @@ struct SystemState {
     DevBuf face, bary, vel, force, mass, energy;
     DevBuf xyz, key, cellCount, cellStart, cellFill, order;
     DevBuf cnt, off, toff, cand, maxd, rad;
-    DevBuf disp, tstatus, gstatus, partial, counters, ke, chain, pstat;
+    DevBuf disp, tstatus, gstatus, partial, counters, ke, chain, pstat, plist, pcount;
     DevBuf scratch;
     ScratchLayout layout{};
     int scratchBlocks = 0;
@@ -61,7 +61,7 @@ struct SystemState {
     std::vector<Mark> marks;
 };
 
-enum MarkKind : int { MARK_NEIGHBORS = 0, MARK_GEODESIC_LDS = 1, MARK_GEODESIC_HBM = 2, MARK_TRANSPORT = 3, MARK_N = 4 };
+enum MarkKind : int { MARK_NEIGHBORS = 0, MARK_GEODESIC_LDS = 1, MARK_GEODESIC_HBM = 2, MARK_TRANSPORT = 3, MARK_PATCH = 4, MARK_N = 5 };
 
 // records an event pair around a group of launches on the context stream when profiling is on
 struct MarkScope {
@@ -85,7 +85,7 @@ struct MarkScope {
 void destroySystem(SystemState* s) {
     DevBuf* all[] = {&s->face, &s->bary, &s->vel, &s->force, &s->mass, &s->energy, &s->xyz, &s->key, &s->cellCount, &s->cellStart,
                      &s->cellFill, &s->order, &s->cnt, &s->off, &s->toff, &s->cand, &s->maxd, &s->rad, &s->disp, &s->tstatus,
-                     &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->scratch};
+                     &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->scratch};
     for (DevBuf* b : all) b->release();
     for (auto& m : s->marks) {
         hipEventDestroy(m.a);
@@ -420,6 +420,13 @@ static csg_status sysForces(csg_ctx* c, SystemState* S) {
     G.sentinelDist = 2.0 * c->maximumDistance;
     G.patchStats = S->pstat.as<int>();
     if (c->submeshing) {
+        G.patchList = S->plist.as<int32_t>();
+        G.patchCount = S->pcount.as<int32_t>();
+        G.patchStride = kPatchListCap;
+        {
+            MarkScope mark(S, c->stream, MARK_PATCH);
+            HIPCHECK(c, launchPatch(G, S->plist.as<int32_t>(), S->pcount.as<int32_t>(), kPatchListCap, c->stream));
+        }
         {
             MarkScope mark(S, c->stream, MARK_GEODESIC_LDS);
             HIPCHECK(c, launchGeodesicLds(G, c->stream));
@@ -503,7 +510,9 @@ csg_status csg_system_init(csg_ctx* c, int32_t N, const int32_t* face, const dou
     HIPCHECK(c, S->disp.ensure(24 * (size_t)N));
     HIPCHECK(c, S->tstatus.ensure(4 * (size_t)N));
     HIPCHECK(c, S->gstatus.ensure(4 * (size_t)N));
-    HIPCHECK(c, S->pstat.ensure(8 * (size_t)N));
+    HIPCHECK(c, S->pstat.ensure(4 * kPatchStats * (size_t)N));
+    HIPCHECK(c, S->plist.ensure(4 * (size_t)kPatchListCap * N));
+    HIPCHECK(c, S->pcount.ensure(4 * (size_t)N));
     HIPCHECK(c, S->partial.ensure(8 * (size_t)blocksFor(N)));
     HIPCHECK(c, S->counters.ensure(4 * CNT_N));
     HIPCHECK(c, S->ke.ensure(16));
@@ -743,22 +752,23 @@ csg_status csg_system_kernel_times(csg_ctx* c, double* ms, int32_t* launches) {
     return CSG_OK;
 }
 
-csg_status csg_system_patch_stats(csg_ctx* c, int64_t* sumFaces, int64_t* sumVerts, int32_t* maxFaces) {
+csg_status csg_system_patch_stats(csg_ctx* c, int64_t* sums, int32_t* maxima) {
     SystemState* S = sysOf(c);
     if (!c) return CSG_ERR_ARGUMENT;
     if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
-    std::vector<int> st(2 * (size_t)S->N);
-    HIPCHECK(c, hipMemcpy(st.data(), S->pstat.p, 8 * (size_t)S->N, hipMemcpyDeviceToHost));
-    int64_t f = 0, v = 0;
-    int mf = 0;
-    for (int i = 0; i < S->N; ++i) {
-        f += st[2 * i];
-        v += st[2 * i + 1];
-        mf = std::max(mf, st[2 * i]);
+    std::vector<int> st(kPatchStats * (size_t)S->N);
+    HIPCHECK(c, hipMemcpy(st.data(), S->pstat.p, 4 * kPatchStats * (size_t)S->N, hipMemcpyDeviceToHost));
+    int64_t sm[kPatchStats] = {};
+    int mx[kPatchStats] = {};
+    for (int i = 0; i < S->N; ++i)
+        for (int a = 0; a < kPatchStats; ++a) {
+            sm[a] += st[kPatchStats * i + a];
+            mx[a] = std::max(mx[a], st[kPatchStats * i + a]);
+        }
+    for (int a = 0; a < kPatchStats; ++a) {
+        if (sums) sums[a] = sm[a];
+        if (maxima) maxima[a] = mx[a];
     }
-    if (sumFaces) *sumFaces = f;
-    if (sumVerts) *sumVerts = v;
-    if (maxFaces) *maxFaces = mf;
     return CSG_OK;
 }
 

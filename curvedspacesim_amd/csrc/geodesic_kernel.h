@@ -9,13 +9,18 @@ namespace csg {
 // face and vertex hash slots (at most 3/4 filled), live work items of the FIFO, targets per source. A source that
 // exceeds any of them is solved again by the scratch variant.
 constexpr int kLdsFaceCap = 1024;
-constexpr int kLdsVertCap = 1024;
+constexpr int kLdsVertCap = 512;
 constexpr int kLdsQueueCap = 2048;
 constexpr int kLdsTargetCap = 256;
 constexpr int kLdsThreads = 256;
+// patch kernel: hash slots per source and capacity of the written face list
+constexpr int kPatchHashCap = 2048;
+constexpr int kPatchListCap = 1024;
 
 // reasons a source exceeded the capacities of a kernel variant (bits of the per-source status)
 enum OverflowReason : int { OVF_FACES = 1, OVF_VERTS = 2, OVF_QUEUE = 4, OVF_TARGETS = 8, OVF_STEPS = 16, OVF_FRONTIER = 32 };
+
+constexpr int kPatchStats = 7;
 
 struct GeoParams {
     DeviceMesh mesh;
@@ -26,6 +31,9 @@ struct GeoParams {
     const int32_t* __restrict__ toff;    // nSrc + 1 offsets into tidx and into the pair outputs
     const int32_t* __restrict__ tidx;    // particle index of each target
     const double* __restrict__ patchRadius;  // per source; nullptr: whole mesh (no submeshing)
+    const int32_t* __restrict__ patchList;   // optional: face sets from the patch kernel, patchStride per source
+    const int32_t* __restrict__ patchCount;  // per source: faces listed, or -1 (the solver then searches itself)
+    int patchStride;
     double rMax;                         // windows beyond this lower-bound distance are not propagated
     // outputs, all optional
     double* pairDist;
@@ -35,7 +43,8 @@ struct GeoParams {
     double* forces;     // 3 per source
     double* energy;     // per source
     int* status;        // per source: 0 solved, else OverflowReason bits
-    int* patchStats;    // per source: patch faces, patch vertices (optional)
+    int* patchStats;    // kPatchStats per source: patch faces, patch vertices, FIFO steps, work items, and the durations
+                        // of patch construction, propagation and the epilogue in 10 ns ticks (optional)
     Potential pot;
     int useSentinel;    // unreachable pairs get (sentinelDist, (0,0,1)) as the reference does
     double sentinelDist;
@@ -48,6 +57,8 @@ struct ScratchLayout {
 };
 
 size_t scratchBytesPerBlock(int fcap, int vcap, int qcap, int tcap);
+// face sets of every source (requires P.patchRadius); stride <= kPatchListCap
+hipError_t launchPatch(const GeoParams& P, int32_t* list, int32_t* count, int stride, hipStream_t stream);
 hipError_t launchGeodesicLds(const GeoParams& P, hipStream_t stream);
 hipError_t launchGeodesicScratch(const GeoParams& P, const ScratchLayout& L, const int32_t* list, int nList, int blocks,
                                  hipStream_t stream);

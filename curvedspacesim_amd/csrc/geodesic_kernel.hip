@@ -52,6 +52,8 @@ struct Tables {
     unsigned long long* vdist;
     float2* vtan;    // start tangent of the best path to the vertex (source-face frame)
     uint8_t* vflg;   // 1: may act as a pseudo-source
+    int32_t* vfo;    // per vertex slot: end of its range in vfl (ranges are consecutive, the first starts at 0)
+    IT* vfl;         // patch faces around each vertex: face slot * 4 + the vertex's corner index
     // FIFO of work items: windows and vertex events
     double *qb0, *qb1, *qsx, *qsy, *qsg;
     int32_t* qfk;    // window: face slot * 8 + k * 2 + rooted (< 2^30); vertex event: -(vertex slot) - 1;
@@ -59,6 +61,7 @@ struct Tables {
     float2* qt;      // rooted window: rotation (cos, sin) to the source-face frame; otherwise its fixed start tangent
     // targets of the source
     IT* tnext;
+    double* tq;      // 3 per target: barycentric weights normalised by their sum
     unsigned long long* tbest;
     float2* tst;     // start tangent (source-face frame)
     float4* tet;     // end tangent (xyz)
@@ -67,10 +70,38 @@ struct Tables {
 };
 
 struct Shared {
-    int head, tail, nFaces, nVerts, overflow, nCur, nNext, goals;
+    int head, tail, nFaces, nVerts, overflow, nCur, nNext, goals, nSteps, nItems;
     double S[3], t1[3], t2[3];
     double red[kMaxWaves][4];
+    int wsum[kMaxWaves];
 };
+
+// Exclusive prefix sum of a[0..n) by the whole workgroup: each lane scans a contiguous chunk, the chunk totals are
+// scanned across the wavefront with lane shuffles and across wavefronts through LDS. Ends with a barrier.
+template <class IT>
+__device__ void blockExclusiveScan(IT* a, int n, Shared& sh) {
+    const int nt = blockDim.x, lane = threadIdx.x;
+    const int chunk = (n + nt - 1) / nt;
+    const int b = min(n, lane * chunk), e = min(n, b + chunk);
+    int sum = 0;
+    for (int i = b; i < e; ++i) sum += a[i];
+    const int wl = lane & 63, w = lane >> 6;
+    int x = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (wl >= o) x += y;
+    }
+    if (wl == 63) sh.wsum[w] = x;
+    __syncthreads();
+    int run = x - sum;
+    for (int k = 0; k < w; ++k) run += sh.wsum[k];
+    for (int i = b; i < e; ++i) {
+        int c = a[i];
+        a[i] = run;
+        run += c;
+    }
+    __syncthreads();
+}
 
 template <class TT>
 __device__ int faceLookup(const TT& T, int g) {
@@ -279,9 +310,8 @@ __device__ void windowTargets(const Ctx<TT>& C, int fs, int g, int k, double L, 
     const TT& T = C.T;
     const GeoParams& P = C.P;
     for (int t = T.fhead[fs]; t >= 0; t = T.tnext[t]) {
-        const double* b = P.pbary + 3 * P.tidx[C.tb + t];
-        double sb = b[0] + b[1] + b[2];
-        double bk1 = b[(k + 1) % 3] / sb, bk2 = b[(k + 2) % 3] / sb;
+        const double* b = T.tq + 3 * t;
+        double bk1 = b[(k + 1) % 3], bk2 = b[(k + 2) % 3];
         double qx = bk1 * L + bk2 * x2, qy = bk2 * y2;
         double rx = qx - sx, ry = qy - sy;
         double ax = b0 - sx, ay = -sy, bx = b1 - sx, by = -sy;
@@ -368,13 +398,12 @@ __device__ void propagate(const Ctx<TT>& C, int fk, int g, double L, double x2, 
 template <class TT>
 __device__ void fanOutVertex(const Ctx<TT>& C, int vslot, double d) {
     const TT& T = C.T;
-    const DeviceMesh& M = C.P.mesh;
-    int v = T.vkey[vslot];
     float2 tv = T.vtan[vslot];
-    for (int q = M.vfoff[v]; q < M.vfoff[v + 1]; ++q) {
-        int e = M.vflist[q];
-        int gs = faceLookup(T, e >> 2);
-        if (gs >= 0 && T.fnb[3 * gs + ((e & 3) + 1) % 3] >= 0) pushVertexFace(C, gs, e & 3, d, tv);
+    const int q0 = vslot > 0 ? T.vfo[vslot - 1] : 0, q1 = T.vfo[vslot];
+    for (int q = q0; q < q1; ++q) {
+        int e = T.vfl[q];
+        int gs = e >> 2, j = e & 3;
+        if (T.fnb[3 * gs + (j + 1) % 3] >= 0) pushVertexFace(C, gs, j, d, tv);
     }
 }
 
@@ -391,7 +420,119 @@ __device__ inline double wsum(double x) {
     return x;
 }
 
+// The overflow bits as every wave of the workgroup sees them: read between two barriers, so that no wave can set a
+// bit (in the phase that follows) before the slowest wave has read them, and all waves take the same branch.
+__device__ inline bool uniformOverflow(const Shared& sh) {
+    __syncthreads();
+    const bool b = sh.overflow != 0;
+    __syncthreads();
+    return b;
+}
+
 // Builds the patch and the per-slot tables. Returns false (with sh.overflow set) when a capacity is exceeded.
+// source point (construct_point semantics: weights normalised by their sum) and the source face's tangent frame
+__device__ void initSourceFrame(const GeoParams& P, Shared& sh, int pi, int sf) {
+    const DeviceMesh& M = P.mesh;
+    const double* bb = P.pbary + 3 * pi;
+    double s = bb[0] + bb[1] + bb[2];
+    double p0[3], p1[3], p2[3];
+    loadV(M, M.fv[3 * sf], p0);
+    loadV(M, M.fv[3 * sf + 1], p1);
+    loadV(M, M.fv[3 * sf + 2], p2);
+    for (int a = 0; a < 3; ++a) sh.S[a] = (p0[a] * bb[0] + p1[a] * bb[1] + p2[a] * bb[2]) / s;
+    double e[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    double l = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    for (int a = 0; a < 3; ++a) sh.t1[a] = e[a] / l;
+    const double* n = M.fnrm + 3 * sf;
+    sh.t2[0] = n[1] * sh.t1[2] - n[2] * sh.t1[1];
+    sh.t2[1] = n[2] * sh.t1[0] - n[0] * sh.t1[2];
+    sh.t2[2] = n[0] * sh.t1[1] - n[1] * sh.t1[0];
+}
+
+// The reference's submesh rule (src/utility/submesher.cpp:55-147): the source face; when some target lies elsewhere, its
+// edge neighbours; when some target lies beyond those, a breadth-first search over faces with at least one corner
+// within the radius of the source point (squared distance not greater than r^2) - the set reached equals the
+// reference's depth-first set, since it is the closure of the same rule; finally every target face.
+template <class TT>
+__device__ void patchSearch(const GeoParams& P, const TT& T, Shared& sh, int* cur, int* nxt, int frontCap, int src, int sf, int tb,
+                            int nT) {
+    const int lane = threadIdx.x;
+    const DeviceMesh& M = P.mesh;
+    const double r = P.patchRadius[src];
+    const double r2 = r * r;
+    int nbr[3];
+    for (int k = 0; k < 3; ++k) nbr[k] = M.fadj[3 * sf + k] < 0 ? -2 : (M.fadj[3 * sf + k] >> 2);
+    int anyGoal = 0, beyond = 0;
+    for (int t = lane; t < nT; t += blockDim.x) {
+        int f = P.pface[P.tidx[tb + t]];
+        if (f != sf) anyGoal = 1;
+        if (f != sf && f != nbr[0] && f != nbr[1] && f != nbr[2]) beyond = 1;
+    }
+    if (anyGoal) atomicOr(&sh.goals, 1);
+    if (beyond) atomicOr(&sh.goals, 2);
+    __syncthreads();
+    if (lane == 0) {
+        bool fresh;
+        faceInsert(T, sh, sf, fresh);
+        if (sh.goals & 1)
+            for (int k = 0; k < 3; ++k)
+                if (nbr[k] >= 0) {
+                    faceInsert(T, sh, nbr[k], fresh);
+                    if (fresh) cur[sh.nCur++] = nbr[k];
+                }
+        if (sh.overflow) sh.nCur = 0;
+    }
+    __syncthreads();
+    if (sh.goals & 2) {
+        // breadth-first: the set reached equals the reference's depth-first set (order-independent closure)
+        while (true) {
+            // nCur is written only between barriers, so every wave takes the same branch
+            const int nc = sh.nCur;
+            if (nc == 0) break;
+            for (int i = lane; i < nc; i += blockDim.x) {
+                int f = cur[i];
+                for (int k = 0; k < 3; ++k) {
+                    int a = M.fadj[3 * f + k];
+                    if (a < 0) continue;
+                    int g = a >> 2;
+                    if (faceLookup(T, g) >= 0) continue;
+                    bool outside = true;
+                    for (int j = 0; j < 3; ++j) {
+                        double p[3];
+                        loadV(M, M.fv[3 * g + j], p);
+                        double dx = p[0] - sh.S[0], dy = p[1] - sh.S[1], dz = p[2] - sh.S[2];
+                        if (!(dx * dx + dy * dy + dz * dz > r2)) outside = false;
+                    }
+                    if (outside) continue;
+                    bool fresh;
+                    int s = faceInsert(T, sh, g, fresh);
+                    if (s >= 0 && fresh) {
+                        int pos = atomicAdd(&sh.nNext, 1);
+                        if (pos < frontCap)
+                            nxt[pos] = g;
+                        else
+                            atomicOr(&sh.overflow, OVF_FRONTIER);
+                    }
+                }
+            }
+            __syncthreads();
+            const int nn = min(sh.nNext, frontCap);
+            for (int i = lane; i < nn; i += blockDim.x) cur[i] = nxt[i];
+            __syncthreads();
+            if (lane == 0) {
+                sh.nCur = sh.overflow ? 0 : nn;
+                sh.nNext = 0;
+            }
+            __syncthreads();
+        }
+    }
+    // target faces not reached above (possibly disconnected from the source)
+    for (int t = lane; t < nT; t += blockDim.x) {
+        bool fresh;
+        faceInsert(T, sh, P.pface[P.tidx[tb + t]], fresh);
+    }
+}
+
 template <class TT>
 __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int src, int sf, int tb, int nT) {
     const int lane = threadIdx.x;
@@ -399,86 +540,23 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int src,
     int* cur = (int*)T.qb0;  // frontier lists borrow the FIFO storage (two ints per double)
     int* nxt = (int*)T.qb1;
     const int frontCap = 2 * T.qcap;
+    const int listed = (P.patchRadius && P.patchList) ? P.patchCount[src] : -1;
     if (P.patchRadius == nullptr) {
         for (int g = lane; g < M.nF; g += blockDim.x) {
             bool fresh;
             faceInsert(T, sh, g, fresh);
         }
+    } else if (listed >= 0) {
+        // face set prepared by the patch kernel
+        const int32_t* lst = P.patchList + (size_t)src * P.patchStride;
+        for (int i = lane; i < listed; i += blockDim.x) {
+            bool fresh;
+            faceInsert(T, sh, lst[i], fresh);
+        }
     } else {
-        const double r = P.patchRadius[src];
-        const double r2 = r * r;
-        int nbr[3];
-        for (int k = 0; k < 3; ++k) nbr[k] = M.fadj[3 * sf + k] < 0 ? -2 : (M.fadj[3 * sf + k] >> 2);
-        int anyGoal = 0, beyond = 0;
-        for (int t = lane; t < nT; t += blockDim.x) {
-            int f = P.pface[P.tidx[tb + t]];
-            if (f != sf) anyGoal = 1;
-            if (f != sf && f != nbr[0] && f != nbr[1] && f != nbr[2]) beyond = 1;
-        }
-        if (anyGoal) atomicOr(&sh.goals, 1);
-        if (beyond) atomicOr(&sh.goals, 2);
-        __syncthreads();
-        if (lane == 0) {
-            bool fresh;
-            faceInsert(T, sh, sf, fresh);
-            if (sh.goals & 1)
-                for (int k = 0; k < 3; ++k)
-                    if (nbr[k] >= 0) {
-                        faceInsert(T, sh, nbr[k], fresh);
-                        if (fresh) cur[sh.nCur++] = nbr[k];
-                    }
-        }
-        __syncthreads();
-        if (sh.goals & 2) {
-            // breadth-first: the set reached equals the reference's depth-first set (order-independent closure)
-            while (true) {
-                const int nc = sh.nCur;
-                if (nc == 0 || sh.overflow) break;
-                for (int i = lane; i < nc; i += blockDim.x) {
-                    int f = cur[i];
-                    for (int k = 0; k < 3; ++k) {
-                        int a = M.fadj[3 * f + k];
-                        if (a < 0) continue;
-                        int g = a >> 2;
-                        if (faceLookup(T, g) >= 0) continue;
-                        bool outside = true;
-                        for (int j = 0; j < 3; ++j) {
-                            double p[3];
-                            loadV(M, M.fv[3 * g + j], p);
-                            double dx = p[0] - sh.S[0], dy = p[1] - sh.S[1], dz = p[2] - sh.S[2];
-                            if (!(dx * dx + dy * dy + dz * dz > r2)) outside = false;
-                        }
-                        if (outside) continue;
-                        bool fresh;
-                        int s = faceInsert(T, sh, g, fresh);
-                        if (s >= 0 && fresh) {
-                            int pos = atomicAdd(&sh.nNext, 1);
-                            if (pos < frontCap)
-                                nxt[pos] = g;
-                            else
-                                atomicOr(&sh.overflow, OVF_FRONTIER);
-                        }
-                    }
-                }
-                __syncthreads();
-                const int nn = min(sh.nNext, frontCap);
-                for (int i = lane; i < nn; i += blockDim.x) cur[i] = nxt[i];
-                __syncthreads();
-                if (lane == 0) {
-                    sh.nCur = nn;
-                    sh.nNext = 0;
-                }
-                __syncthreads();
-            }
-        }
-        // target faces not reached above (possibly disconnected from the source)
-        for (int t = lane; t < nT; t += blockDim.x) {
-            bool fresh;
-            faceInsert(T, sh, P.pface[P.tidx[tb + t]], fresh);
-        }
+        patchSearch(P, T, sh, cur, nxt, frontCap, src, sf, tb, nT);
     }
-    __syncthreads();
-    if (sh.overflow) return false;
+    if (uniformOverflow(sh)) return false;
     for (int s = lane; s <= T.fmask; s += blockDim.x) {
         int g = T.fkey[s];
         if (g < 0) continue;
@@ -489,24 +567,32 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int src,
             T.fnb[3 * s + k] = ns < 0 ? -1 : ns * 4 + (a & 3);
         }
     }
+    if (uniformOverflow(sh)) return false;
+    // vertex -> patch faces table (for pseudo-source fan-out): counts, exclusive prefix sum, fill
+    for (int s = lane; s <= T.fmask; s += blockDim.x)
+        if (T.fkey[s] >= 0)
+            for (int k = 0; k < 3; ++k) atomicAdd(&T.vfo[T.fcv[3 * s + k]], 1);
     __syncthreads();
-    if (sh.overflow) return false;
+    // pseudo-source candidates: saddle and mesh-border vertices, and vertices with an incident face outside the patch
     for (int s = lane; s <= T.vmask; s += blockDim.x) {
         int v = T.vkey[s];
         if (v < 0) continue;
-        int fl = (M.vflag[v] & 3) ? 1 : 0;
-        if (!fl)
-            for (int q = M.vfoff[v]; q < M.vfoff[v + 1]; ++q)
-                if (faceLookup(T, M.vflist[q] >> 2) < 0) {
-                    fl = 1;
-                    break;
-                }
-        T.vflg[s] = fl;
+        T.vflg[s] = ((M.vflag[v] & 3) || T.vfo[s] < M.vfoff[v + 1] - M.vfoff[v]) ? 1 : 0;
         T.vdist[s] = dbits(kInf);
         T.vtan[s] = make_float2(0.f, 0.f);
     }
-    // per-face target lists: slots looked up in parallel (parked in tbest), linked by one lane
-    for (int t = lane; t < nT; t += blockDim.x) T.tbest[t] = (unsigned long long)faceLookup(T, P.pface[P.tidx[tb + t]]);
+    blockExclusiveScan(T.vfo, T.vmask + 1, sh);
+    for (int s = lane; s <= T.fmask; s += blockDim.x)
+        if (T.fkey[s] >= 0)
+            for (int k = 0; k < 3; ++k) T.vfl[atomicAdd(&T.vfo[T.fcv[3 * s + k]], 1)] = (s * 4 + k);
+    // per-face target lists: slots looked up in parallel (parked in tbest), linked by one lane; normalised weights
+    for (int t = lane; t < nT; t += blockDim.x) {
+        const int pj = P.tidx[tb + t];
+        T.tbest[t] = (unsigned long long)faceLookup(T, P.pface[pj]);
+        const double* b = P.pbary + 3 * pj;
+        double sb = b[0] + b[1] + b[2];
+        for (int a = 0; a < 3; ++a) T.tq[3 * t + a] = b[a] / sb;
+    }
     __syncthreads();
     if (lane == 0)
         for (int t = nT - 1; t >= 0; --t) {
@@ -564,9 +650,12 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
         if (T.vflg[vs] && d <= C.rMax) pushVertexEvent(C, vs, d);
     }
     __syncthreads();
+    int h = 0;  // every lane advances its own copy of the head
     for (int step = 0;; ++step) {
-        const int h = sh.head, t = sh.tail;
-        if (h >= t || sh.overflow) break;
+        const int t = sh.tail;
+        const bool stop = h >= t || sh.overflow;
+        __syncthreads();  // every wave has read the tail and the flags before any wave appends or sets one
+        if (stop) break;
         if (step >= kMaxSteps) {  // every source ends: an unfinished one is reported like a capacity overflow
             if (lane == 0) atomicOr(&sh.overflow, OVF_STEPS);
             break;
@@ -606,7 +695,7 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
                 y2 = geo[2];
             }
         }
-        __syncthreads();
+        // no barrier here: appends of this step land at positions >= t, never on the records [h, h + nb) just read
         VCand cP0 = noCand(), cP1 = noCand(), cP2 = noCand();
         bool live = false;
         if (mine) {
@@ -636,8 +725,11 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
             finishVertexCand(C, cP2);
         }
         __syncthreads();
-        if (lane == 0) sh.head = h + nb;
-        __syncthreads();
+        h += nb;
+        if (lane == 0) {
+            sh.nSteps += 1;
+            sh.nItems += nb;
+        }
     }
 }
 
@@ -652,33 +744,26 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
         T.fkey[i] = -1;
         T.fhead[i] = -1;
     }
-    for (int i = lane; i <= T.vmask; i += blockDim.x) T.vkey[i] = -1;
+    for (int i = lane; i <= T.vmask; i += blockDim.x) {
+        T.vkey[i] = -1;
+        T.vfo[i] = 0;
+    }
     for (int i = lane; i < min(nT, T.tcap); i += blockDim.x) T.tbest[i] = dbits(kInf);
     if (lane == 0) {
-        sh.head = sh.tail = sh.nFaces = sh.nVerts = 0;
+        sh.head = sh.tail = sh.nFaces = sh.nVerts = sh.nSteps = sh.nItems = 0;
         sh.overflow = nT > T.tcap ? OVF_TARGETS : 0;
         sh.nCur = sh.nNext = sh.goals = 0;
-        const double* bb = P.pbary + 3 * pi;
-        double s = bb[0] + bb[1] + bb[2];
-        double p0[3], p1[3], p2[3];
-        loadV(M, M.fv[3 * sf], p0);
-        loadV(M, M.fv[3 * sf + 1], p1);
-        loadV(M, M.fv[3 * sf + 2], p2);
-        for (int a = 0; a < 3; ++a) sh.S[a] = (p0[a] * bb[0] + p1[a] * bb[1] + p2[a] * bb[2]) / s;
-        double e[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
-        double l = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
-        for (int a = 0; a < 3; ++a) sh.t1[a] = e[a] / l;
-        const double* n = M.fnrm + 3 * sf;
-        sh.t2[0] = n[1] * sh.t1[2] - n[2] * sh.t1[1];
-        sh.t2[1] = n[2] * sh.t1[0] - n[0] * sh.t1[2];
-        sh.t2[2] = n[0] * sh.t1[1] - n[1] * sh.t1[0];
+        initSourceFrame(P, sh, pi, sf);
     }
     __syncthreads();
     bool ok = !sh.overflow;
     __syncthreads();
+    const long long c0 = wall_clock64();
     if (ok) ok = buildPatch(P, T, sh, src, sf, tb, nT);
+    const long long c1 = wall_clock64();
     if (ok) propagateAll(P, T, sh, sf, tb);
     __syncthreads();
+    const long long c2 = wall_clock64();
     // ---- finalize: paths through the target face's corners, direct visibility, outputs, force
     const bool ovf = sh.overflow != 0;
     double F0 = 0, F1 = 0, F2 = 0, E = 0;
@@ -799,9 +884,15 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
         }
         if (P.energy) P.energy[src] = E;
         if (P.status) P.status[src] = sh.overflow;
-        if (P.patchStats) {
-            P.patchStats[2 * src] = sh.nFaces;
-            P.patchStats[2 * src + 1] = sh.nVerts;
+        if (P.patchStats) {  // counts, then phase durations in ticks of the constant 100 MHz clock
+            int* ps = P.patchStats + kPatchStats * src;
+            ps[0] = sh.nFaces;
+            ps[1] = sh.nVerts;
+            ps[2] = sh.nSteps;
+            ps[3] = sh.nItems;
+            ps[4] = (int)(c1 - c0);
+            ps[5] = (int)(c2 - c1);
+            ps[6] = (int)(wall_clock64() - c2);
         }
     }
     __syncthreads();
@@ -814,14 +905,16 @@ template <int FH, int VH, int QC, int TC, int NT>
 __global__ __launch_bounds__(NT) void geodesicForceLds(GeoParams P) {
     __shared__ int32_t fkey[FH];
     __shared__ int16_t fhead[FH], fcv[3 * FH], fnb[3 * FH];
-    __shared__ int32_t vkey[VH];
+    __shared__ int32_t vkey[VH], vfo[VH];
     __shared__ uint8_t vflg[VH];
+    __shared__ int16_t vfl[3 * ((FH * 3) / 4)];
     __shared__ unsigned long long vdist[VH];
     __shared__ float2 vtan[VH];
     __shared__ double qb0[QC], qb1[QC], qsx[QC], qsy[QC], qsg[QC];
     __shared__ int32_t qfk[QC];
     __shared__ float2 qt[QC];
     __shared__ int16_t tnext[TC];
+    __shared__ double tq[3 * TC];
     __shared__ unsigned long long tbest[TC];
     __shared__ float2 tst[TC];
     __shared__ float4 tet[TC];
@@ -829,7 +922,7 @@ __global__ __launch_bounds__(NT) void geodesicForceLds(GeoParams P) {
     static_assert(2 * QC >= FH / 2, "BFS frontiers live in the FIFO storage");
     static_assert(4 * FH <= 32767 && VH <= 32767 && TC <= 32767, "slots must fit int16");
     static_assert(NT % 64 == 0 && NT / 64 <= kMaxWaves, "whole wavefronts");
-    Tables<int16_t> T{fkey, fcv, fnb, fhead, vkey, vdist, vtan, vflg, qb0, qb1, qsx, qsy, qsg, qfk, qt, tnext, tbest, tst, tet,
+    Tables<int16_t> T{fkey, fcv, fnb, fhead, vkey, vdist, vtan, vflg, vfo, vfl, qb0, qb1, qsx, qsy, qsg, qfk, qt, tnext, tq, tbest, tst, tet,
              FH - 1, VH - 1, QC, TC, (FH * 3) / 4, (VH * 3) / 4};
     const int src = blockIdx.x;
     if (src >= P.nSrc) return;
@@ -854,6 +947,8 @@ __global__ __launch_bounds__(64) void geodesicForceScratch(GeoParams P, ScratchL
     T.fnb = (int32_t*)take(12 * (size_t)L.fcap);
     T.vkey = (int32_t*)take(4 * (size_t)L.vcap);
     T.vflg = (uint8_t*)take((size_t)L.vcap);
+    T.vfo = (int32_t*)take(4 * (size_t)L.vcap);
+    T.vfl = (int32_t*)take(12 * (size_t)L.fcap);
     T.vdist = (unsigned long long*)take(8 * (size_t)L.vcap);
     T.vtan = (float2*)take(8 * (size_t)L.vcap);
     T.qb0 = (double*)take(8 * (size_t)L.qcap);
@@ -864,6 +959,7 @@ __global__ __launch_bounds__(64) void geodesicForceScratch(GeoParams P, ScratchL
     T.qfk = (int32_t*)take(4 * (size_t)L.qcap);
     T.qt = (float2*)take(8 * (size_t)L.qcap);
     T.tnext = (int32_t*)take(4 * (size_t)L.tcap);
+    T.tq = (double*)take(24 * (size_t)L.tcap);
     T.tbest = (unsigned long long*)take(8 * (size_t)L.tcap);
     T.tst = (float2*)take(8 * (size_t)L.tcap);
     T.tet = (float4*)take(16 * (size_t)L.tcap);
@@ -880,10 +976,57 @@ __global__ __launch_bounds__(64) void geodesicForceScratch(GeoParams P, ScratchL
     }
 }
 
+// Patch kernel: the reference's submesh face set of every source, one wavefront and ~16 KB of LDS per source (so many
+// sources per CU hide the search's memory latency), written as a compact list (count -1: more than the list holds).
+struct FaceSet {
+    int32_t* fkey;
+    int fmask, flimit;
+};
+
+__global__ __launch_bounds__(64) void patchKernel(GeoParams P, int32_t* list, int32_t* count, int stride) {
+    __shared__ int32_t fkey[kPatchHashCap];
+    __shared__ int32_t cur[kPatchListCap], nxt[kPatchListCap];
+    __shared__ Shared sh;
+    const int src = blockIdx.x;
+    if (src >= P.nSrc) return;
+    const int lane = threadIdx.x;
+    const int pi = P.srcIdx ? P.srcIdx[src] : src;
+    const int sf = P.pface[pi];
+    const int tb = P.toff[src], nT = P.toff[src + 1] - tb;
+    FaceSet T{fkey, kPatchHashCap - 1, min(stride, (kPatchHashCap * 3) / 4)};
+    for (int i = lane; i < kPatchHashCap; i += blockDim.x) fkey[i] = -1;
+    if (lane == 0) {
+        sh.nFaces = sh.overflow = sh.nCur = sh.nNext = sh.goals = 0;
+        initSourceFrame(P, sh, pi, sf);
+    }
+    __syncthreads();
+    patchSearch(P, T, sh, cur, nxt, kPatchListCap, src, sf, tb, nT);
+    __syncthreads();
+    if (sh.overflow) {
+        if (lane == 0) count[src] = -1;
+        return;
+    }
+    if (lane == 0) sh.nCur = 0;
+    __syncthreads();
+    int32_t* out = list + (size_t)src * stride;
+    for (int i = lane; i < kPatchHashCap; i += blockDim.x)
+        if (fkey[i] >= 0) out[atomicAdd(&sh.nCur, 1)] = fkey[i];
+    __syncthreads();
+    if (lane == 0) count[src] = sh.nCur;
+}
+
+hipError_t launchPatch(const GeoParams& P, int32_t* list, int32_t* count, int stride, hipStream_t stream) {
+    if (P.nSrc <= 0 || P.patchRadius == nullptr) return hipSuccess;
+    if (stride > kPatchListCap) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(patchKernel, dim3(P.nSrc), dim3(64), 0, stream, P, list, count, stride);
+    return hipGetLastError();
+}
+
 size_t scratchBytesPerBlock(int fcap, int vcap, int qcap, int tcap) {
     auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    return 2 * r(4 * (size_t)fcap) + 2 * r(12 * (size_t)fcap) + 2 * r(4 * (size_t)vcap) + 2 * r(8 * (size_t)vcap) + 5 * r(8 * (size_t)qcap) +
-           r(4 * (size_t)qcap) + r(8 * (size_t)qcap) + r(4 * (size_t)tcap) + 2 * r(8 * (size_t)tcap) + r(16 * (size_t)tcap);
+    return 2 * r(4 * (size_t)fcap) + 3 * r(12 * (size_t)fcap) + 3 * r(4 * (size_t)vcap) + 2 * r(8 * (size_t)vcap) + 5 * r(8 * (size_t)qcap) +
+           r(4 * (size_t)qcap) + r(8 * (size_t)qcap) + r(4 * (size_t)tcap) + r(24 * (size_t)tcap) + 2 * r(8 * (size_t)tcap) +
+           r(16 * (size_t)tcap);
 }
 
 hipError_t launchGeodesicLds(const GeoParams& P, hipStream_t stream) {

@@ -129,13 +129,15 @@ csg_status csg_system_get(csg_ctx* ctx, int32_t* face, double* bary, double* vel
 csg_status csg_system_status(csg_ctx* ctx, int64_t* pairsLastForce, int32_t* candOverflow, int32_t* unsolved, int32_t* walkFailures,
                              double* kineticEnergy);
 /* event timing of the step's launches (no reference counterpart; replaces the wall-clock profiler of
- * src/utility/profiler.h:16-59): when enabled, HIP events on the context stream bracket each launch group; ms[4] and
- * launches[4] receive the totals since the last query for {neighbour search, geodesic LDS kernel, geodesic HBM
- * kernel, walk kernel} */
+ * src/utility/profiler.h:16-59): when enabled, HIP events on the context stream bracket each launch group; ms[5] and
+ * launches[5] receive the totals since the last query for {neighbour search, geodesic LDS kernel, geodesic HBM
+ * kernel, walk kernel, patch kernel} */
 csg_status csg_system_profile(csg_ctx* ctx, int32_t enable);
 csg_status csg_system_kernel_times(csg_ctx* ctx, double* ms, int32_t* launches);
-/* patch sizes of the last force computation: sums over sources of patch faces and vertices, largest patch */
-csg_status csg_system_patch_stats(csg_ctx* ctx, int64_t* sumFaces, int64_t* sumVerts, int32_t* maxFaces);
+/* work statistics of the last force computation: sums[7] and maxima[7] over sources of {patch faces, patch vertices,
+ * FIFO steps, work items (windows and pseudo-source items), and the durations of patch construction, propagation and
+ * epilogue in 10 ns ticks}; either pointer nullable */
+csg_status csg_system_patch_stats(csg_ctx* ctx, int64_t* sums, int32_t* maxima);
 /* sources handed from the LDS kernel to the HBM kernel since csg_system_init, by reason: reasons[6] = {patch faces,
  * patch vertices, live windows, targets, step limit, search frontier} */
 csg_status csg_system_fallback_counts(csg_ctx* ctx, int32_t* reasons);
