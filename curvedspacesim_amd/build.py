@@ -9,7 +9,10 @@ ROOT = os.path.dirname(PKG)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["csg_api.hip", "csg_system.hip", "geodesic_kernel.hip", "transport_kernel.hip"]
-HOST_SOURCES = ["host/meshGenerators.cpp", "host/hostCapi.cpp"]
+HOST_SOURCES = ["host/meshGenerators.cpp", "host/hostCapi.cpp", "host/noiseSource.cpp", "host/gpuMeshSpace.cpp",
+                "host/neighborStructures.cpp", "host/simpleModel.cpp", "host/forces.cpp", "host/simulation.cpp"]
+# reference-style scenario driver of the C++ host classes, run by the parity tests
+SCENARIOS = os.path.join(ROOT, "tests", "cpp", "refapi_scenarios.cpp")
 # packed FP32 instructions are kept out of every code object (see the hipcc feature flag below)
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -40,12 +43,22 @@ def build_host():
     return out
 
 
+def build_scenarios():
+    out = os.path.join(ROOT, "tests", "cpp", "refapi_scenarios")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(CSRC, "host"), SCENARIOS, "-L" + PKG, "-lcsg_host", "-lcsg",
+           "-Wl,-rpath," + os.path.relpath(PKG, os.path.dirname(out)).join(["$ORIGIN/", ""]), "-o", out + ".tmp"]
+    _run(cmd, CSRC)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all(verbose=False):
     a = build_csg()
     b = build_host()
+    c = build_scenarios()
     if verbose:
-        print("built", a, b)
-    return a, b
+        print("built", a, b, c)
+    return a, b, c
 
 
 if __name__ == "__main__":

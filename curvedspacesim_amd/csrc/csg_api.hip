@@ -316,6 +316,7 @@ csg_status csg_pair_forces(csg_ctx* c, int32_t nP, const int32_t* face, const do
     G.pot = pot ? csg::Potential{pot->kind, pot->p0, pot->p1, pot->p2} : csg::Potential{csg::POT_NONE, 0, 0, 0};
     G.useSentinel = useSentinel;
     G.sentinelDist = sentinelDistance;
+    G.rMax = csg::exactTruncation(G.pot, rMax, useSentinel != 0, sentinelDistance);
     csg_status s = runGeodesics(c, G, maxT, patchRadius == nullptr, true);
     if (s) return s;
     HIPCHECK(c, download(c, dist, c->pdist, (size_t)nPairs));

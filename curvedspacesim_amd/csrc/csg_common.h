@@ -59,4 +59,23 @@ __host__ __device__ inline double potentialScale(const Potential& p, double d, d
     return sc;
 }
 
+// Distance beyond which the potential gives no force and no energy (infinite for the Gaussian, 0 for none).
+__host__ __device__ inline double potentialRange(const Potential& p) {
+    if (p.kind == POT_HARMONIC) return p.p1;
+    if (p.kind == POT_TRUNC_GAUSSIAN) return p.p2;
+    if (p.kind == POT_GAUSSIAN) return 1.0 / 0.0;
+    return 0.0;
+}
+
+// Truncation radius that keeps every force and energy exact. Windows beyond rMax are dropped, so a pair farther than
+// rMax ends up unreached and, with the sentinel on, is given the sentinel distance. That changes nothing when the
+// potential vanishes both beyond rMax and at the sentinel distance; otherwise truncation is switched off.
+inline double exactTruncation(const Potential& p, double rMax, bool useSentinel, double sentinelDist) {
+    if (p.kind == POT_NONE) return rMax;
+    const double range = potentialRange(p);
+    if (rMax < range) rMax = range;
+    if (useSentinel && sentinelDist < range) return 1.0 / 0.0;
+    return rMax;
+}
+
 }  // namespace csg

@@ -418,6 +418,7 @@ static csg_status sysForces(csg_ctx* c, SystemState* S) {
     G.pot = S->pot;
     G.useSentinel = c->submeshing;
     G.sentinelDist = 2.0 * c->maximumDistance;
+    G.rMax = exactTruncation(S->pot, S->rMax, G.useSentinel != 0, G.sentinelDist);
     G.patchStats = S->pstat.as<int>();
     if (c->submeshing) {
         G.patchList = S->plist.as<int32_t>();

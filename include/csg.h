@@ -79,7 +79,9 @@ csg_status csg_distance_batch(csg_ctx* ctx, int32_t nSrc, const int32_t* srcFace
  * src/forces/baseForce.cpp:12-44) over explicit neighbour lists of particles given by position -------------------
  * Particles: nP positions. Source s is particle srcIdx[s] (nullable: s) with candidate particles
  * tgtIdx[tgtOffsets[s] .. tgtOffsets[s+1]-1]; patchRadius (nullable: whole mesh) per source; windows whose distance
- * lower bound exceeds rMax are not propagated (pass a huge value for none). Unreached pairs get the reference
+ * lower bound exceeds rMax are not propagated (pass a huge value for none; with a potential, rMax is raised to the
+ * potential's range, and truncation is switched off when the sentinel distance lies inside that range, so that
+ * forces and energies stay exact; only the reported distances of pairs beyond rMax change). Unreached pairs get the reference
  * sentinel when useSentinel != 0. forces: 3 per source, sum over its pairs of pairForce(startTangent, distance);
  * energy: per source sum of pair energies. Every output is nullable. */
 csg_status csg_pair_forces(csg_ctx* ctx, int32_t nP, const int32_t* face, const double* bary, int32_t nSrc, const int32_t* srcIdx,
