@@ -127,13 +127,7 @@ int pow2AtLeast(long n) {
 
 // scratch tables sized for `faces` patch faces and `targets` targets per source
 csg_status ensureScratch(csg_ctx* c, long faces, long targets, int blocks) {
-    csg::ScratchLayout L{};
-    L.fcap = pow2AtLeast(std::max(2 * faces + 2, 1024L));
-    L.vcap = pow2AtLeast(std::max(2 * faces + 16, 512L));
-    L.qcap = pow2AtLeast(std::max(16 * faces, 4096L));
-    L.qcap = std::max(L.qcap, L.fcap);  // frontier lists borrow the FIFO
-    L.tcap = (int)std::max(targets, 1L);
-    L.bytesPerBlock = csg::scratchBytesPerBlock(L.fcap, L.vcap, L.qcap, L.tcap);
+    csg::ScratchLayout L = csg::makeScratchLayout(faces, targets);
     size_t need = L.bytesPerBlock * (size_t)blocks;
     HIPCHECK(c, c->scratch.ensure(need));
     L.base = c->scratch.p;
@@ -154,6 +148,9 @@ csg_status runGeodesics(csg_ctx* c, csg::GeoParams& G, int maxTargets, bool whol
             G.patchList = c->plist.as<int32_t>();
             G.patchCount = c->pcount.as<int32_t>();
             G.patchStride = csg::kPatchListCap;
+            HIPCHECK(c, c->spill.ensure(sizeof(csg::SpillRecord) * (size_t)csg::kSpillCap * G.nSrc));
+            G.spill = c->spill.as<csg::SpillRecord>();
+            G.spillCap = csg::kSpillCap;
             HIPCHECK(c, csg::launchPatch(G, c->plist.as<int32_t>(), c->pcount.as<int32_t>(), G.patchStride, c->stream));
         }
         HIPCHECK(c, csg::launchGeodesicLds(G, c->stream));

@@ -56,7 +56,7 @@ struct csg_ctx {
     int submeshing = 0;
     double maximumDistance = 0;
     // per-call staging
-    DevBuf pface, pbary, srcIdx, toff, tidx, prad, pdist, pst, pet, pflags, pforces, penergy, status, list, plist, pcount;
+    DevBuf pface, pbary, srcIdx, toff, tidx, prad, pdist, pst, pet, pflags, pforces, penergy, status, list, plist, pcount, spill;
     DevBuf tdisp, tvecs, tstat;
     DevBuf scratch;
     int scratchBlocks = 0;
@@ -65,7 +65,7 @@ struct csg_ctx {
     ~csg_ctx() {
         if (sys) csg::destroySystem(sys);
         DevBuf* all[] = {&vx, &fv, &fadj, &fgeo, &fnrm, &vflag, &vfoff, &vflist, &pface, &pbary, &srcIdx, &toff, &tidx, &prad,
-                         &pdist, &pst, &pet, &pflags, &pforces, &penergy, &status, &list, &plist, &pcount, &tdisp, &tvecs, &tstat, &scratch};
+                         &pdist, &pst, &pet, &pflags, &pforces, &penergy, &status, &list, &plist, &pcount, &spill, &tdisp, &tvecs, &tstat, &scratch};
         for (DevBuf* b : all) b->release();
         if (stream) hipStreamDestroy(stream);
     }

@@ -35,7 +35,7 @@ struct SystemState {
     DevBuf face, bary, vel, force, mass, energy;
     DevBuf xyz, key, cellCount, cellStart, cellFill, order;
     DevBuf cnt, off, toff, cand, maxd, rad;
-    DevBuf disp, tstatus, gstatus, partial, counters, ke, chain, pstat, plist, pcount;
+    DevBuf disp, tstatus, gstatus, partial, counters, ke, chain, pstat, plist, pcount, spill;
     DevBuf scratch;
     ScratchLayout layout{};
     int scratchBlocks = 0;
@@ -85,7 +85,7 @@ struct MarkScope {
 void destroySystem(SystemState* s) {
     DevBuf* all[] = {&s->face, &s->bary, &s->vel, &s->force, &s->mass, &s->energy, &s->xyz, &s->key, &s->cellCount, &s->cellStart,
                      &s->cellFill, &s->order, &s->cnt, &s->off, &s->toff, &s->cand, &s->maxd, &s->rad, &s->disp, &s->tstatus,
-                     &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->scratch};
+                     &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->spill, &s->scratch};
     for (DevBuf* b : all) b->release();
     for (auto& m : s->marks) {
         hipEventDestroy(m.a);
@@ -420,10 +420,16 @@ static csg_status sysForces(csg_ctx* c, SystemState* S) {
     G.sentinelDist = 2.0 * c->maximumDistance;
     G.rMax = exactTruncation(S->pot, S->rMax, G.useSentinel != 0, G.sentinelDist);
     G.patchStats = S->pstat.as<int>();
+    if (S->cellList) {  // sources in cell order, one contiguous chunk per XCD
+        G.blockOrder = S->order.as<int32_t>();
+        G.blockChunk = (S->N + kXcds - 1) / kXcds;
+    }
     if (c->submeshing) {
         G.patchList = S->plist.as<int32_t>();
         G.patchCount = S->pcount.as<int32_t>();
         G.patchStride = kPatchListCap;
+        G.spill = S->spill.as<SpillRecord>();
+        G.spillCap = kSpillCap;
         {
             MarkScope mark(S, c->stream, MARK_PATCH);
             HIPCHECK(c, launchPatch(G, S->plist.as<int32_t>(), S->pcount.as<int32_t>(), kPatchListCap, c->stream));
@@ -514,6 +520,7 @@ csg_status csg_system_init(csg_ctx* c, int32_t N, const int32_t* face, const dou
     HIPCHECK(c, S->pstat.ensure(4 * kPatchStats * (size_t)N));
     HIPCHECK(c, S->plist.ensure(4 * (size_t)kPatchListCap * N));
     HIPCHECK(c, S->pcount.ensure(4 * (size_t)N));
+    HIPCHECK(c, S->spill.ensure(sizeof(SpillRecord) * (size_t)kSpillCap * N));
     HIPCHECK(c, S->partial.ensure(8 * (size_t)blocksFor(N)));
     HIPCHECK(c, S->counters.ensure(4 * CNT_N));
     HIPCHECK(c, S->ke.ensure(16));
@@ -560,17 +567,7 @@ csg_status csg_system_configure(csg_ctx* c, const csg_potential* pot, double ran
     long faces = c->submeshing ? std::max<int>(maxPatchFaces > 0 ? maxPatchFaces : 1536, 1024) : c->nF;
     long maxT = c->submeshing && S->cellList ? std::min<long>(S->candCap, 4096) : N;
     int blocks = c->submeshing ? 1024 : std::min(N, 256);
-    ScratchLayout L{};
-    auto p2 = [](long n) {
-        int p = 1;
-        while (p < n) p <<= 1;
-        return p;
-    };
-    L.fcap = p2(std::max(2 * faces + 2, 1024L));
-    L.vcap = p2(std::max(2 * faces + 16, 512L));
-    L.qcap = std::max(p2(std::max(16 * faces, 4096L)), L.fcap);
-    L.tcap = (int)std::max(maxT, 1L);
-    L.bytesPerBlock = scratchBytesPerBlock(L.fcap, L.vcap, L.qcap, L.tcap);
+    ScratchLayout L = makeScratchLayout(faces, maxT);
     HIPCHECK(c, S->scratch.ensure(L.bytesPerBlock * (size_t)blocks));
     L.base = S->scratch.p;
     S->layout = L;

@@ -5,14 +5,20 @@
 
 namespace csg {
 
-// LDS capacities of the fast variant (one source per workgroup of kLdsThreads lanes, ~150 KB of the CU's 160 KB):
-// face and vertex hash slots (at most 3/4 filled), live work items of the FIFO, targets per source. A source that
-// exceeds any of them is solved again by the scratch variant.
-constexpr int kLdsFaceCap = 1024;
-constexpr int kLdsVertCap = 512;
-constexpr int kLdsQueueCap = 2048;
+// Capacities of the LDS variant (one source per workgroup of kLdsThreads lanes, under 80 KB so that two workgroups
+// share a CU): patch faces and vertices, records of the work-item ring, targets per source, and the slots of the
+// setup-time face and vertex hashes. A source that exceeds the first, second or fourth, or its spill stack, is solved
+// again by the scratch variant.
+constexpr int kLdsFaceCap = 640;
+constexpr int kLdsVertCap = 384;
+constexpr int kLdsQueueCap = 768;
 constexpr int kLdsTargetCap = 256;
+constexpr int kLdsFaceHash = 1024;
+constexpr int kLdsVertHash = 1024;
 constexpr int kLdsThreads = 256;
+// work items beyond the ring wait in a per-source HBM stack of this many records
+constexpr int kSpillCap = 1024;
+constexpr int kXcds = 8;  // accelerator complex dies of an MI355X, each with its own L2
 // patch kernel: hash slots per source and capacity of the written face list
 constexpr int kPatchHashCap = 2048;
 constexpr int kPatchListCap = 1024;
@@ -21,6 +27,14 @@ constexpr int kPatchListCap = 1024;
 enum OverflowReason : int { OVF_FACES = 1, OVF_VERTS = 2, OVF_QUEUE = 4, OVF_TARGETS = 8, OVF_STEPS = 16, OVF_FRONTIER = 32 };
 
 constexpr int kPatchStats = 7;
+
+// one spilled work item (the fields of a ring record)
+struct SpillRecord {
+    double b0, b1, sx, sy, sg;
+    int32_t fk;
+    float tx, ty;
+    int32_t pad;
+};
 
 struct GeoParams {
     DeviceMesh mesh;
@@ -31,9 +45,13 @@ struct GeoParams {
     const int32_t* __restrict__ toff;    // nSrc + 1 offsets into tidx and into the pair outputs
     const int32_t* __restrict__ tidx;    // particle index of each target
     const double* __restrict__ patchRadius;  // per source; nullptr: whole mesh (no submeshing)
+    const int32_t* __restrict__ blockOrder;  // optional: source order (spatially sorted), split over the XCDs in chunks
+    int blockChunk;                          // ceil(nSrc / kXcds) when blockOrder is set
     const int32_t* __restrict__ patchList;   // optional: face sets from the patch kernel, patchStride per source
     const int32_t* __restrict__ patchCount;  // per source: faces listed, or -1 (the solver then searches itself)
     int patchStride;
+    SpillRecord* spill;                  // optional: spillCap records per source for work items beyond the LDS ring
+    int spillCap;
     double rMax;                         // windows beyond this lower-bound distance are not propagated
     // outputs, all optional
     double* pairDist;
@@ -53,10 +71,13 @@ struct GeoParams {
 struct ScratchLayout {
     void* base;
     size_t bytesPerBlock;
-    int fcap, vcap, qcap, tcap;  // powers of two (tcap any)
+    int fcap, vcap, qcap, tcap;  // faces, vertices, ring records, targets per source
+    int hfcap, hvcap;            // face and vertex hash slots (powers of two)
 };
 
-size_t scratchBytesPerBlock(int fcap, int vcap, int qcap, int tcap);
+// capacities and bytes per workgroup of the scratch variant for sources of up to `faces` patch faces and `targets`
+// targets (base is left null)
+ScratchLayout makeScratchLayout(long faces, long targets);
 // face sets of every source (requires P.patchRadius); stride <= kPatchListCap
 hipError_t launchPatch(const GeoParams& P, int32_t* list, int32_t* count, int stride, hipStream_t stream);
 hipError_t launchGeodesicLds(const GeoParams& P, hipStream_t stream);

@@ -1,11 +1,15 @@
 // Batched exact polyhedral geodesics + pair forces on gfx950.
 //
-// One workgroup (one 64-wide wavefront) solves one source particle:
-//   1. patch   : the reference's submesh rule (src/utility/submesher.cpp:55-147) as a breadth-first search over
-//                faces into an LDS hash table (global face id -> slot); every slot caches its corners' vertex slots and
-//                its neighbours' face slots, so the propagation touches HBM only for face geometry and coordinates.
+// One workgroup (four 64-wide wavefronts) solves one source particle; two workgroups share a CU, so that one's waves
+// run while the other's wait at a barrier or on memory:
+//   1. patch   : the face set of the reference's submesh rule (src/utility/submesher.cpp:55-147), prepared by the
+//                patch kernel, is numbered into compact LDS slots through setup-time hash tables (global id -> slot);
+//                every face slot caches its corners' vertex slots and its neighbours' face slots, so the propagation
+//                touches HBM only for face geometry and coordinates.
 //   2. windows : exact window (interval) propagation in FP64 (the method behind CGAL's Surface_mesh_shortest_path,
-//                src/utility/meshUtilities.h:44-48), up to 64 work items per step from an LDS FIFO, one per lane.
+//                src/utility/meshUtilities.h:44-48), up to 256 work items per step from an LDS ring, one per lane;
+//                items that do not fit the ring wait in a per-source HBM spill stack and are moved back when the ring
+//                drains (the result does not depend on the order in which windows are processed).
 //                Vertex distance bounds are LDS 64-bit atomic minima on the IEEE bit pattern (distances are
 //                non-negative), and a window is dropped when those bounds show it cannot carry a shortest path
 //                anywhere on its interval (Xin & Wang's filter). Saddle vertices (angle sum >= 2 pi) and vertices on
@@ -17,10 +21,12 @@
 //                so no path is traced back (reference: computePathDistanceAndTangents,
 //                src/utility/meshUtilities.cpp:360-380).
 //   4. forces  : the pair force (src/forces/baseForce.cpp:12-28) summed per source with a wavefront reduction.
-// A source whose patch, vertex set, target list or live FIFO exceeds the LDS capacities is flagged and solved again
-// by the same code with its tables in a per-workgroup HBM scratch slice; that variant also serves the whole-mesh
-// mode of triangulatedMeshSpace::distance (src/models/triangulatedMeshSpace.cpp:222-237).
+// A source whose patch, vertex set, target list or spill exceeds the capacities is flagged and solved again by the
+// same code with its tables in a per-workgroup HBM scratch slice; that variant also serves the whole-mesh mode of
+// triangulatedMeshSpace::distance (src/models/triangulatedMeshSpace.cpp:222-237).
 #include <math.h>
+
+#include <algorithm>
 
 #include "csg_common.h"
 #include "geodesic_kernel.h"
@@ -40,37 +46,45 @@ __device__ inline unsigned long long dbits(double d) { return (unsigned long lon
 __device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 __device__ inline uint32_t hashOf(uint32_t key) { return (key * 2654435761u) >> 7; }
 
-// Tables of one source. LDS variant: every pointer refers to __shared__ storage; scratch variant: to a slice of a
-// global buffer. Hash capacities are powers of two.
+// Tables of one source. LDS variant: every pointer refers to __shared__ storage (the setup hashes share the FIFO's
+// storage, which is idle until the propagation starts); scratch variant: to a slice of a global buffer. Faces and
+// vertices are numbered 0, 1, ... in insertion order (compact slots); hash capacities are powers of two.
 template <typename IT>
 struct Tables {
-    int32_t* fkey;   // face hash: global face id or -1
+    int32_t* hkey;   // face hash: global face id or -1
+    IT* hslot;       // face hash: compact slot of the key
+    int32_t* vhkey;  // vertex hash: global vertex id or -1
+    IT* vhslot;      // vertex hash: compact slot of the key
+    int32_t* fglob;  // per face slot: global face id
     IT* fcv;         // 3 per face slot: corner vertex slots
     IT* fnb;         // 3 per face slot: neighbour face slot * 4 + its local edge index, or -1 outside the patch
     IT* fhead;       // per face slot: first target lying in that face, or -1
-    int32_t* vkey;   // vertex hash: global vertex id or -1
+    int32_t* vglob;  // per vertex slot: global vertex id
     unsigned long long* vdist;
     float2* vtan;    // start tangent of the best path to the vertex (source-face frame)
     uint8_t* vflg;   // 1: may act as a pseudo-source
     int32_t* vfo;    // per vertex slot: end of its range in vfl (ranges are consecutive, the first starts at 0)
     IT* vfl;         // patch faces around each vertex: face slot * 4 + the vertex's corner index
-    // FIFO of work items: windows and vertex events
+    // FIFO ring of work items: windows and vertex events
     double *qb0, *qb1, *qsx, *qsy, *qsg;
     int32_t* qfk;    // window: face slot * 8 + k * 2 + rooted (< 2^30); vertex event: -(vertex slot) - 1;
                      // vertex-face item: kVertexFace | (face slot * 4 + corner of the vertex)
     float2* qt;      // rooted window: rotation (cos, sin) to the source-face frame; otherwise its fixed start tangent
     // targets of the source
     IT* tnext;
+    IT* tslot;       // face slot of the target
     double* tq;      // 3 per target: barycentric weights normalised by their sum
     unsigned long long* tbest;
     float2* tst;     // start tangent (source-face frame)
-    float4* tet;     // end tangent (xyz)
-    int fmask, vmask, qcap, tcap;
-    int flimit, vlimit;
+    float4* tet;     // end tangent (xyz); nullptr in variants that do not report it
+    int hmask, vhmask;           // hash capacities - 1
+    int fcap, vcap, qcap, tcap;  // compact faces, compact vertices, ring records, targets
 };
 
 struct Shared {
     int head, tail, nFaces, nVerts, overflow, nCur, nNext, goals, nSteps, nItems;
+    int spillTop;  // records in the source's HBM spill stack
+    int srcSlot;   // face slot of the source face
     double S[3], t1[3], t2[3];
     double red[kMaxWaves][4];
     int wsum[kMaxWaves];
@@ -103,66 +117,95 @@ __device__ void blockExclusiveScan(IT* a, int n, Shared& sh) {
     __syncthreads();
 }
 
+// Face hash. Lookups after a barrier that follows the last insertion see every key and slot; during an insertion
+// phase only the presence of a key (faceKnown) is reliable.
+template <class TT>
+__device__ bool faceKnown(const TT& T, int g) {
+    uint32_t h = hashOf((uint32_t)g) & T.hmask;
+    for (int p = 0; p <= T.hmask; ++p) {
+        int k = T.hkey[h];
+        if (k == g) return true;
+        if (k == -1) return false;
+        h = (h + 1) & T.hmask;
+    }
+    return false;
+}
+
 template <class TT>
 __device__ int faceLookup(const TT& T, int g) {
-    uint32_t h = hashOf((uint32_t)g) & T.fmask;
-    for (int p = 0; p <= T.fmask; ++p) {
-        int k = T.fkey[h];
-        if (k == g) return (int)h;
+    uint32_t h = hashOf((uint32_t)g) & T.hmask;
+    for (int p = 0; p <= T.hmask; ++p) {
+        int k = T.hkey[h];
+        if (k == g) return (int)T.hslot[h];
         if (k == -1) return -1;
-        h = (h + 1) & T.fmask;
+        h = (h + 1) & T.hmask;
     }
     return -1;
 }
 
-// slot of g (inserting it if needed), or -1 past the fill limit; `fresh` reports whether this call inserted it
+// inserts g (if absent) and gives it the next compact slot; `fresh` reports whether this call inserted it. Past the
+// face capacity the overflow bit is set.
 template <class TT>
-__device__ int faceInsert(const TT& T, Shared& sh, int g, bool& fresh) {
+__device__ void faceInsert(const TT& T, Shared& sh, int g, bool& fresh) {
     fresh = false;
-    uint32_t h = hashOf((uint32_t)g) & T.fmask;
-    for (int p = 0; p <= T.fmask; ++p) {
-        int k = T.fkey[h];
-        if (k == g) return (int)h;
+    uint32_t h = hashOf((uint32_t)g) & T.hmask;
+    for (int p = 0; p <= T.hmask; ++p) {
+        int k = T.hkey[h];
+        if (k == g) return;
         if (k == -1) {
-            if (atomicAdd(&sh.nFaces, 1) >= T.flimit) {
-                atomicAdd(&sh.nFaces, -1);
-                atomicOr(&sh.overflow, OVF_FACES);
-                return -1;
-            }
-            int old = atomicCAS(&T.fkey[h], -1, g);
+            int old = atomicCAS(&T.hkey[h], -1, g);
             if (old == -1) {
+                int slot = atomicAdd(&sh.nFaces, 1);
+                if (slot >= T.fcap) {
+                    atomicOr(&sh.overflow, OVF_FACES);
+                    return;
+                }
+                T.hslot[h] = slot;
+                T.fglob[slot] = g;
                 fresh = true;
-                return (int)h;
+                return;
             }
-            atomicAdd(&sh.nFaces, -1);
-            if (old == g) return (int)h;
+            if (old == g) return;
         }
-        h = (h + 1) & T.fmask;
+        h = (h + 1) & T.hmask;
     }
     atomicOr(&sh.overflow, OVF_FACES);
-    return -1;
 }
 
 template <class TT>
-__device__ int vertInsert(const TT& T, Shared& sh, int v) {
-    uint32_t h = hashOf((uint32_t)v) & T.vmask;
-    for (int p = 0; p <= T.vmask; ++p) {
-        int k = T.vkey[h];
-        if (k == v) return (int)h;
+__device__ void vertInsert(const TT& T, Shared& sh, int v) {
+    uint32_t h = hashOf((uint32_t)v) & T.vhmask;
+    for (int p = 0; p <= T.vhmask; ++p) {
+        int k = T.vhkey[h];
+        if (k == v) return;
         if (k == -1) {
-            if (atomicAdd(&sh.nVerts, 1) >= T.vlimit) {
-                atomicAdd(&sh.nVerts, -1);
-                atomicOr(&sh.overflow, OVF_VERTS);
-                return -1;
+            int old = atomicCAS(&T.vhkey[h], -1, v);
+            if (old == -1) {
+                int slot = atomicAdd(&sh.nVerts, 1);
+                if (slot >= T.vcap) {
+                    atomicOr(&sh.overflow, OVF_VERTS);
+                    return;
+                }
+                T.vhslot[h] = slot;
+                T.vglob[slot] = v;
+                return;
             }
-            int old = atomicCAS(&T.vkey[h], -1, v);
-            if (old == -1) return (int)h;
-            atomicAdd(&sh.nVerts, -1);
-            if (old == v) return (int)h;
+            if (old == v) return;
         }
-        h = (h + 1) & T.vmask;
+        h = (h + 1) & T.vhmask;
     }
     atomicOr(&sh.overflow, OVF_VERTS);
+}
+
+template <class TT>
+__device__ int vertLookup(const TT& T, int v) {
+    uint32_t h = hashOf((uint32_t)v) & T.vhmask;
+    for (int p = 0; p <= T.vhmask; ++p) {
+        int k = T.vhkey[h];
+        if (k == v) return (int)T.vhslot[h];
+        if (k == -1) return -1;
+        h = (h + 1) & T.vhmask;
+    }
     return -1;
 }
 
@@ -235,7 +278,48 @@ struct Ctx {
     int qBase;  // head at the start of the step: appends stay below qBase + qcap, so live records are never overwritten
     double rMax;
     int tb;     // first pair index of the source
+    SpillRecord* spill;  // this source's HBM spill stack (nullptr: none)
 };
+
+// Appends one work item: to the LDS ring while it has room, else to the source's HBM spill stack. A ring position at
+// or past qBase + qcap is handed back at once, so the positions below it are handed out exactly once.
+template <class TT>
+__device__ void pushItem(const Ctx<TT>& C, double b0, double b1, double sx, double sy, double sg, int32_t fk, float2 t) {
+    const TT& T = C.T;
+    int pos = atomicAdd(&C.sh.tail, 1);
+    if (pos - C.qBase < T.qcap) {
+        int q = pos % T.qcap;
+        T.qb0[q] = b0;
+        T.qb1[q] = b1;
+        T.qsx[q] = sx;
+        T.qsy[q] = sy;
+        T.qsg[q] = sg;
+        T.qfk[q] = fk;
+        T.qt[q] = t;
+        return;
+    }
+    atomicSub(&C.sh.tail, 1);
+    if (C.spill) {
+        int sp = atomicAdd(&C.sh.spillTop, 1);
+        if (sp >= C.P.spillCap) {
+            atomicSub(&C.sh.spillTop, 1);  // as for the ring: entries past the capacity are handed back
+        } else {
+            SpillRecord r;
+            r.b0 = b0;
+            r.b1 = b1;
+            r.sx = sx;
+            r.sy = sy;
+            r.sg = sg;
+            r.fk = fk;
+            r.tx = t.x;
+            r.ty = t.y;
+            r.pad = 0;
+            C.spill[sp] = r;
+            return;
+        }
+    }
+    atomicOr(&C.sh.overflow, OVF_QUEUE);
+}
 
 // A new window on (face slot fs, local edge k): distance candidates for reached edge endpoints, filter, append.
 template <class TT>
@@ -249,44 +333,17 @@ __device__ void addWindow(const Ctx<TT>& C, int fs, int k, double L, double b0, 
     if (b0 <= tol) offerVertex(T, cA, vA, sg + sqrt(sx * sx + sy * sy), startTangent(rooted, rot, -sx, -sy));
     if (b1 >= L - tol) offerVertex(T, cB, vB, sg + sqrt((L - sx) * (L - sx) + sy * sy), startTangent(rooted, rot, L - sx, -sy));
     if (filtered(T, vA, vB, L, b0, b1, sx, sy, sg)) return;
-    int pos = atomicAdd(&C.sh.tail, 1);
-    if (pos - C.qBase >= T.qcap) {
-        atomicOr(&C.sh.overflow, OVF_QUEUE);
-        return;
-    }
-    int q = pos & (T.qcap - 1);
-    T.qb0[q] = b0;
-    T.qb1[q] = b1;
-    T.qsx[q] = sx;
-    T.qsy[q] = sy;
-    T.qsg[q] = sg;
-    T.qfk[q] = fs * 8 + k * 2 + (rooted ? 1 : 0);
-    T.qt[q] = rot;
+    pushItem(C, b0, b1, sx, sy, sg, fs * 8 + k * 2 + (rooted ? 1 : 0), rot);
 }
 
 template <class TT>
 __device__ inline void pushVertexEvent(const Ctx<TT>& C, int vslot, double d) {
-    int pos = atomicAdd(&C.sh.tail, 1);
-    if (pos - C.qBase >= C.T.qcap) {
-        atomicOr(&C.sh.overflow, OVF_QUEUE);
-        return;
-    }
-    int q = pos & (C.T.qcap - 1);
-    C.T.qsg[q] = d;
-    C.T.qfk[q] = -vslot - 1;
+    pushItem(C, 0.0, 0.0, 0.0, 0.0, d, -vslot - 1, make_float2(0.f, 0.f));
 }
 
 template <class TT>
 __device__ inline void pushVertexFace(const Ctx<TT>& C, int gs, int j, double d, float2 tv) {
-    int pos = atomicAdd(&C.sh.tail, 1);
-    if (pos - C.qBase >= C.T.qcap) {
-        atomicOr(&C.sh.overflow, OVF_QUEUE);
-        return;
-    }
-    int q = pos & (C.T.qcap - 1);
-    C.T.qsg[q] = d;
-    C.T.qt[q] = tv;
-    C.T.qfk[q] = kVertexFace | (gs * 4 + j);
+    pushItem(C, 0.0, 0.0, 0.0, 0.0, d, kVertexFace | (gs * 4 + j), tv);
 }
 
 // pseudo-source point Q (global coordinates) in the frame of the window entering face g through its edge k
@@ -495,7 +552,7 @@ __device__ void patchSearch(const GeoParams& P, const TT& T, Shared& sh, int* cu
                     int a = M.fadj[3 * f + k];
                     if (a < 0) continue;
                     int g = a >> 2;
-                    if (faceLookup(T, g) >= 0) continue;
+                    if (faceKnown(T, g)) continue;
                     bool outside = true;
                     for (int j = 0; j < 3; ++j) {
                         double p[3];
@@ -505,8 +562,8 @@ __device__ void patchSearch(const GeoParams& P, const TT& T, Shared& sh, int* cu
                     }
                     if (outside) continue;
                     bool fresh;
-                    int s = faceInsert(T, sh, g, fresh);
-                    if (s >= 0 && fresh) {
+                    faceInsert(T, sh, g, fresh);
+                    if (fresh) {
                         int pos = atomicAdd(&sh.nNext, 1);
                         if (pos < frontCap)
                             nxt[pos] = g;
@@ -533,85 +590,95 @@ __device__ void patchSearch(const GeoParams& P, const TT& T, Shared& sh, int* cu
     }
 }
 
+// Builds the patch and the per-slot tables. Returns false (with sh.overflow set) when a capacity is exceeded. cur/nxt
+// are breadth-first frontier lists for a source without a prepared face list (nullptr: such a source overflows).
 template <class TT>
-__device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int src, int sf, int tb, int nT) {
+__device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int* cur, int* nxt, int frontCap, int src, int sf, int tb,
+                           int nT) {
     const int lane = threadIdx.x;
     const DeviceMesh& M = P.mesh;
-    int* cur = (int*)T.qb0;  // frontier lists borrow the FIFO storage (two ints per double)
-    int* nxt = (int*)T.qb1;
-    const int frontCap = 2 * T.qcap;
+    // A. the face set
     const int listed = (P.patchRadius && P.patchList) ? P.patchCount[src] : -1;
     if (P.patchRadius == nullptr) {
         for (int g = lane; g < M.nF; g += blockDim.x) {
             bool fresh;
             faceInsert(T, sh, g, fresh);
         }
-    } else if (listed >= 0) {
-        // face set prepared by the patch kernel
+    } else if (listed >= 0) {  // prepared by the patch kernel
         const int32_t* lst = P.patchList + (size_t)src * P.patchStride;
         for (int i = lane; i < listed; i += blockDim.x) {
             bool fresh;
             faceInsert(T, sh, lst[i], fresh);
         }
-    } else {
+    } else if (cur) {
         patchSearch(P, T, sh, cur, nxt, frontCap, src, sf, tb, nT);
+    } else if (lane == 0) {
+        atomicOr(&sh.overflow, OVF_FACES);
     }
     if (uniformOverflow(sh)) return false;
-    for (int s = lane; s <= T.fmask; s += blockDim.x) {
-        int g = T.fkey[s];
-        if (g < 0) continue;
+    const int nF = sh.nFaces;
+    // B. their corner vertices
+    for (int s = lane; s < nF; s += blockDim.x) {
+        const int g = T.fglob[s];
+        for (int k = 0; k < 3; ++k) vertInsert(T, sh, M.fv[3 * g + k]);
+    }
+    if (uniformOverflow(sh)) return false;
+    const int nV = sh.nVerts;
+    // C. per face slot: corner and neighbour slots, corner counts per vertex; per target: face slot and weights
+    for (int s = lane; s < nF; s += blockDim.x) {
+        const int g = T.fglob[s];
         for (int k = 0; k < 3; ++k) {
-            T.fcv[3 * s + k] = vertInsert(T, sh, M.fv[3 * g + k]);
-            int a = M.fadj[3 * g + k];
-            int ns = a < 0 ? -1 : faceLookup(T, a >> 2);
+            const int v = vertLookup(T, M.fv[3 * g + k]);
+            T.fcv[3 * s + k] = v;
+            atomicAdd(&T.vfo[v], 1);
+            const int a = M.fadj[3 * g + k];
+            const int ns = a < 0 ? -1 : faceLookup(T, a >> 2);
             T.fnb[3 * s + k] = ns < 0 ? -1 : ns * 4 + (a & 3);
         }
+        T.fhead[s] = -1;
     }
-    if (uniformOverflow(sh)) return false;
-    // vertex -> patch faces table (for pseudo-source fan-out): counts, exclusive prefix sum, fill
-    for (int s = lane; s <= T.fmask; s += blockDim.x)
-        if (T.fkey[s] >= 0)
-            for (int k = 0; k < 3; ++k) atomicAdd(&T.vfo[T.fcv[3 * s + k]], 1);
-    __syncthreads();
-    // pseudo-source candidates: saddle and mesh-border vertices, and vertices with an incident face outside the patch
-    for (int s = lane; s <= T.vmask; s += blockDim.x) {
-        int v = T.vkey[s];
-        if (v < 0) continue;
-        T.vflg[s] = ((M.vflag[v] & 3) || T.vfo[s] < M.vfoff[v + 1] - M.vfoff[v]) ? 1 : 0;
-        T.vdist[s] = dbits(kInf);
-        T.vtan[s] = make_float2(0.f, 0.f);
-    }
-    blockExclusiveScan(T.vfo, T.vmask + 1, sh);
-    for (int s = lane; s <= T.fmask; s += blockDim.x)
-        if (T.fkey[s] >= 0)
-            for (int k = 0; k < 3; ++k) T.vfl[atomicAdd(&T.vfo[T.fcv[3 * s + k]], 1)] = (s * 4 + k);
-    // per-face target lists: slots looked up in parallel (parked in tbest), linked by one lane; normalised weights
     for (int t = lane; t < nT; t += blockDim.x) {
         const int pj = P.tidx[tb + t];
-        T.tbest[t] = (unsigned long long)faceLookup(T, P.pface[pj]);
+        T.tslot[t] = faceLookup(T, P.pface[pj]);
         const double* b = P.pbary + 3 * pj;
         double sb = b[0] + b[1] + b[2];
         for (int a = 0; a < 3; ++a) T.tq[3 * t + a] = b[a] / sb;
+        T.tbest[t] = dbits(kInf);
     }
-    __syncthreads();
+    if (lane == 0) {
+        sh.srcSlot = faceLookup(T, sf);
+        if (sh.srcSlot < 0) atomicOr(&sh.overflow, OVF_FACES);  // the patch rule always holds the source face
+    }
+    if (uniformOverflow(sh)) return false;
+    // D. pseudo-source candidates (saddle and mesh-border vertices, and vertices with an incident face outside the
+    // patch) and the vertices' initial state; then the vertex -> face ranges
+    for (int s = lane; s < nV; s += blockDim.x) {
+        const int v = T.vglob[s];
+        T.vflg[s] = ((M.vflag[v] & 3) || (int)T.vfo[s] < M.vfoff[v + 1] - M.vfoff[v]) ? 1 : 0;
+        T.vdist[s] = dbits(kInf);
+        T.vtan[s] = make_float2(0.f, 0.f);
+    }
+    blockExclusiveScan(T.vfo, nV, sh);
+    // E. the vertex -> face lists; targets linked to their faces by one lane (fixed order)
+    for (int s = lane; s < nF; s += blockDim.x)
+        for (int k = 0; k < 3; ++k) T.vfl[atomicAdd(&T.vfo[T.fcv[3 * s + k]], 1)] = s * 4 + k;
     if (lane == 0)
         for (int t = nT - 1; t >= 0; --t) {
-            int s = (int)T.tbest[t];
+            const int s = T.tslot[t];
+            if (s < 0) continue;
             T.tnext[t] = T.fhead[s];
             T.fhead[s] = t;
         }
-    __syncthreads();
-    for (int t = lane; t < nT; t += blockDim.x) T.tbest[t] = dbits(kInf);
     __syncthreads();
     return true;
 }
 
 template <class TT>
-__device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf, int tb) {
+__device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf, int tb, SpillRecord* spill) {
     const int lane = threadIdx.x;
     const DeviceMesh& M = P.mesh;
-    Ctx<TT> C{P, T, sh, 0, P.rMax, tb};
-    const int ss = faceLookup(T, sf);
+    Ctx<TT> C{P, T, sh, 0, P.rMax, tb, spill};
+    const int ss = sh.srcSlot;
     // corners of the source face are visible directly
     if (lane < 3) {
         double p[3];
@@ -628,7 +695,7 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
     if (lane < 3) {
         int nb = T.fnb[3 * ss + lane];
         if (nb >= 0) {
-            int hs = nb >> 2, kh = nb & 3, h = T.fkey[hs];
+            int hs = nb >> 2, kh = nb & 3, h = T.fglob[hs];
             double sx, sy, L;
             pointInFrame(M, h, kh, sh.S, sx, sy, L);
             double o[3], e[3];
@@ -653,17 +720,39 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
     int h = 0;  // every lane advances its own copy of the head
     for (int step = 0;; ++step) {
         const int t = sh.tail;
-        const bool stop = h >= t || sh.overflow;
+        const int sp = sh.spillTop;
+        const bool ovf = sh.overflow != 0;
         __syncthreads();  // every wave has read the tail and the flags before any wave appends or sets one
-        if (stop) break;
+        if (ovf || (h >= t && sp == 0)) break;
         if (step >= kMaxSteps) {  // every source ends: an unfinished one is reported like a capacity overflow
             if (lane == 0) atomicOr(&sh.overflow, OVF_STEPS);
             break;
         }
+        if (h >= t) {  // the ring is empty: the newest spilled records (at most a ring's worth) move back into it
+            const int m = min(sp, T.qcap);
+            for (int i = lane; i < m; i += blockDim.x) {
+                const SpillRecord r = spill[sp - m + i];
+                const int q = (t + i) % T.qcap;
+                T.qb0[q] = r.b0;
+                T.qb1[q] = r.b1;
+                T.qsx[q] = r.sx;
+                T.qsy[q] = r.sy;
+                T.qsg[q] = r.sg;
+                T.qfk[q] = r.fk;
+                T.qt[q] = make_float2(r.tx, r.ty);
+            }
+            __syncthreads();
+            if (lane == 0) {
+                sh.tail = t + m;
+                sh.spillTop = sp - m;
+            }
+            __syncthreads();
+            continue;
+        }
         const int nb = min((int)blockDim.x, t - h);
         C.qBase = h;
         const bool mine = lane < nb;
-        const int q = (h + lane) & (T.qcap - 1);
+        const int q = (h + lane) % T.qcap;
         int fk = -1, kind = 0, g = -1, fs = 0, k = 0;  // kind: 0 window, 1 vertex event, 2 vertex-face item
         double b0 = 0, b1 = 0, sx = 0, sy = 0, sg = 0, L = 0, x2 = 0, y2 = 0;
         float2 rot = make_float2(0.f, 0.f);
@@ -688,7 +777,7 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
                 kind = 1;
             }
             if (kind != 1) {  // geometry of the edge the item works on, fetched before the barrier
-                g = T.fkey[fs];
+                g = T.fglob[fs];
                 const double* geo = M.fgeo + 9 * g + 3 * k;
                 L = geo[0];
                 x2 = geo[1];
@@ -734,34 +823,30 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
 }
 
 template <class TT>
-__device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src) {
+__device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src, int* cur, int* nxt, int frontCap) {
     const int lane = threadIdx.x;
     const DeviceMesh& M = P.mesh;
     const int pi = P.srcIdx ? P.srcIdx[src] : src;
     const int sf = P.pface[pi];
     const int tb = P.toff[src], nT = P.toff[src + 1] - tb;
-    for (int i = lane; i <= T.fmask; i += blockDim.x) {
-        T.fkey[i] = -1;
-        T.fhead[i] = -1;
-    }
-    for (int i = lane; i <= T.vmask; i += blockDim.x) {
-        T.vkey[i] = -1;
-        T.vfo[i] = 0;
-    }
-    for (int i = lane; i < min(nT, T.tcap); i += blockDim.x) T.tbest[i] = dbits(kInf);
+    for (int i = lane; i <= T.hmask; i += blockDim.x) T.hkey[i] = -1;
+    for (int i = lane; i <= T.vhmask; i += blockDim.x) T.vhkey[i] = -1;
+    for (int i = lane; i < T.vcap; i += blockDim.x) T.vfo[i] = 0;
     if (lane == 0) {
-        sh.head = sh.tail = sh.nFaces = sh.nVerts = sh.nSteps = sh.nItems = 0;
+        sh.head = sh.tail = sh.nFaces = sh.nVerts = sh.nSteps = sh.nItems = sh.spillTop = 0;
+        sh.srcSlot = -1;
         sh.overflow = nT > T.tcap ? OVF_TARGETS : 0;
         sh.nCur = sh.nNext = sh.goals = 0;
         initSourceFrame(P, sh, pi, sf);
     }
+    SpillRecord* spill = P.spill ? P.spill + (size_t)src * P.spillCap : nullptr;
     __syncthreads();
     bool ok = !sh.overflow;
     __syncthreads();
     const long long c0 = wall_clock64();
-    if (ok) ok = buildPatch(P, T, sh, src, sf, tb, nT);
+    if (ok) ok = buildPatch(P, T, sh, cur, nxt, frontCap, src, sf, tb, nT);
     const long long c1 = wall_clock64();
-    if (ok) propagateAll(P, T, sh, sf, tb);
+    if (ok) propagateAll(P, T, sh, sf, tb, spill);
     __syncthreads();
     const long long c2 = wall_clock64();
     // ---- finalize: paths through the target face's corners, direct visibility, outputs, force
@@ -783,7 +868,7 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
         for (int a = 0; a < 3; ++a) Q[a] = (cc[0][a] * b[0] + cc[1][a] * b[1] + cc[2][a] * b[2]) / sb;
         double best = bitsd(T.tbest[t]);
         float2 st = T.tst[t];
-        float4 et4 = T.tet[t];
+        float4 et4 = T.tet ? T.tet[t] : make_float4(0.f, 0.f, 0.f, 0.f);
         double et[3] = {et4.x, et4.y, et4.z};
         if (f == sf) {
             double r[3] = {Q[0] - sh.S[0], Q[1] - sh.S[1], Q[2] - sh.S[2]};
@@ -800,7 +885,7 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
                 }
             }
         }
-        const int fs = faceLookup(T, f);
+        const int fs = T.tslot[t];
         for (int k = 0; k < 3 && fs >= 0; ++k) {
             int vs = T.fcv[3 * fs + k];
             double dv = bitsd(T.vdist[vs]);
@@ -900,33 +985,80 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
 
 }  // namespace
 
-// LDS variant: tables in __shared__ memory, one source per workgroup
-template <int FH, int VH, int QC, int TC, int NT>
-__global__ __launch_bounds__(NT) void geodesicForceLds(GeoParams P) {
-    __shared__ int32_t fkey[FH];
-    __shared__ int16_t fhead[FH], fcv[3 * FH], fnb[3 * FH];
-    __shared__ int32_t vkey[VH], vfo[VH];
-    __shared__ uint8_t vflg[VH];
-    __shared__ int16_t vfl[3 * ((FH * 3) / 4)];
-    __shared__ unsigned long long vdist[VH];
-    __shared__ float2 vtan[VH];
-    __shared__ double qb0[QC], qb1[QC], qsx[QC], qsy[QC], qsg[QC];
-    __shared__ int32_t qfk[QC];
-    __shared__ float2 qt[QC];
-    __shared__ int16_t tnext[TC];
+// Source of this workgroup. Workgroups are dispatched round-robin over the 8 XCDs (workgroup i to XCD i mod 8), each
+// with its own L2. With an order (sources sorted by spatial cell), XCD x takes the contiguous chunk
+// [x * chunk, (x + 1) * chunk) of that order, so the mesh data its sources touch is a compact region that stays in its
+// L2. Returns -1 for the padding workgroups past the last source.
+__device__ inline int sourceOfBlock(const GeoParams& P) {
+    const int i = blockIdx.x;
+    if (P.blockOrder == nullptr) return i < P.nSrc ? i : -1;
+    const int p = (i % kXcds) * P.blockChunk + i / kXcds;
+    return p < P.nSrc ? P.blockOrder[p] : -1;
+}
+
+int gridForSources(const GeoParams& P) { return P.blockOrder ? kXcds * P.blockChunk : P.nSrc; }
+
+// LDS variant: tables in __shared__ memory, one source per workgroup, two workgroups per CU (at most 80 KB each).
+// FC faces, VC vertices, QC ring records, TC targets; HF and HV face and vertex hash slots, which share the ring's
+// storage; kEnd: the variant that also reports end tangents.
+template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, bool kEnd>
+__global__ __launch_bounds__(NT, 2) void geodesicForceLds(GeoParams P) {
+    constexpr int kRingBytes = QC * (5 * 8 + 4 + 8);
+    static_assert(HF * 6 + HV * 6 <= kRingBytes, "setup hashes live in the ring's storage");
+    static_assert(QC % 2 == 0 && (HF & (HF - 1)) == 0 && (HV & (HV - 1)) == 0, "alignment and power-of-two hashes");
+    static_assert(4 * FC <= 32767 && VC <= 32767 && TC <= 32767, "slots must fit int16");
+    static_assert(NT % 64 == 0 && NT / 64 <= kMaxWaves, "whole wavefronts");
+    __shared__ __attribute__((aligned(16))) unsigned char ring[kRingBytes];
+    __shared__ int32_t fglob[FC];
+    __shared__ int16_t fcv[3 * FC], fnb[3 * FC], fhead[FC], vfl[3 * FC];
+    __shared__ int32_t vglob[VC];
+    __shared__ unsigned long long vdist[VC];
+    __shared__ float2 vtan[VC];
+    __shared__ uint8_t vflg[VC];
+    __shared__ int32_t vfo[VC];
+    __shared__ int16_t tnext[TC], tslot[TC];
     __shared__ double tq[3 * TC];
     __shared__ unsigned long long tbest[TC];
     __shared__ float2 tst[TC];
-    __shared__ float4 tet[TC];
+    __shared__ float4 tet[kEnd ? TC : 1];
     __shared__ Shared sh;
-    static_assert(2 * QC >= FH / 2, "BFS frontiers live in the FIFO storage");
-    static_assert(4 * FH <= 32767 && VH <= 32767 && TC <= 32767, "slots must fit int16");
-    static_assert(NT % 64 == 0 && NT / 64 <= kMaxWaves, "whole wavefronts");
-    Tables<int16_t> T{fkey, fcv, fnb, fhead, vkey, vdist, vtan, vflg, vfo, vfl, qb0, qb1, qsx, qsy, qsg, qfk, qt, tnext, tq, tbest, tst, tet,
-             FH - 1, VH - 1, QC, TC, (FH * 3) / 4, (VH * 3) / 4};
-    const int src = blockIdx.x;
-    if (src >= P.nSrc) return;
-    solveSource(P, T, sh, src);
+    Tables<int16_t> T;
+    T.qb0 = (double*)ring;
+    T.qb1 = T.qb0 + QC;
+    T.qsx = T.qb1 + QC;
+    T.qsy = T.qsx + QC;
+    T.qsg = T.qsy + QC;
+    T.qfk = (int32_t*)(T.qsg + QC);
+    T.qt = (float2*)(T.qfk + QC);
+    T.hkey = (int32_t*)ring;
+    T.hslot = (int16_t*)(T.hkey + HF);
+    T.vhkey = (int32_t*)(T.hslot + HF);
+    T.vhslot = (int16_t*)(T.vhkey + HV);
+    T.fglob = fglob;
+    T.fcv = fcv;
+    T.fnb = fnb;
+    T.fhead = fhead;
+    T.vglob = vglob;
+    T.vdist = vdist;
+    T.vtan = vtan;
+    T.vflg = vflg;
+    T.vfo = vfo;
+    T.vfl = vfl;
+    T.tnext = tnext;
+    T.tslot = tslot;
+    T.tq = tq;
+    T.tbest = tbest;
+    T.tst = tst;
+    T.tet = kEnd ? tet : nullptr;
+    T.hmask = HF - 1;
+    T.vhmask = HV - 1;
+    T.fcap = FC;
+    T.vcap = VC;
+    T.qcap = QC;
+    T.tcap = TC;
+    const int src = sourceOfBlock(P);
+    if (src < 0) return;
+    solveSource(P, T, sh, src, nullptr, nullptr, 0);
 }
 
 // Scratch variant: the same solver with its tables in a per-workgroup slice of an HBM buffer, persistent over a
@@ -941,14 +1073,18 @@ __global__ __launch_bounds__(64) void geodesicForceScratch(GeoParams P, ScratchL
         return p;
     };
     Tables<int32_t> T;
-    T.fkey = (int32_t*)take(4 * (size_t)L.fcap);
+    T.hkey = (int32_t*)take(4 * (size_t)L.hfcap);
+    T.hslot = (int32_t*)take(4 * (size_t)L.hfcap);
+    T.vhkey = (int32_t*)take(4 * (size_t)L.hvcap);
+    T.vhslot = (int32_t*)take(4 * (size_t)L.hvcap);
+    T.fglob = (int32_t*)take(4 * (size_t)L.fcap);
     T.fhead = (int32_t*)take(4 * (size_t)L.fcap);
     T.fcv = (int32_t*)take(12 * (size_t)L.fcap);
     T.fnb = (int32_t*)take(12 * (size_t)L.fcap);
-    T.vkey = (int32_t*)take(4 * (size_t)L.vcap);
+    T.vfl = (int32_t*)take(12 * (size_t)L.fcap);
+    T.vglob = (int32_t*)take(4 * (size_t)L.vcap);
     T.vflg = (uint8_t*)take((size_t)L.vcap);
     T.vfo = (int32_t*)take(4 * (size_t)L.vcap);
-    T.vfl = (int32_t*)take(12 * (size_t)L.fcap);
     T.vdist = (unsigned long long*)take(8 * (size_t)L.vcap);
     T.vtan = (float2*)take(8 * (size_t)L.vcap);
     T.qb0 = (double*)take(8 * (size_t)L.qcap);
@@ -959,42 +1095,52 @@ __global__ __launch_bounds__(64) void geodesicForceScratch(GeoParams P, ScratchL
     T.qfk = (int32_t*)take(4 * (size_t)L.qcap);
     T.qt = (float2*)take(8 * (size_t)L.qcap);
     T.tnext = (int32_t*)take(4 * (size_t)L.tcap);
+    T.tslot = (int32_t*)take(4 * (size_t)L.tcap);
     T.tq = (double*)take(24 * (size_t)L.tcap);
     T.tbest = (unsigned long long*)take(8 * (size_t)L.tcap);
     T.tst = (float2*)take(8 * (size_t)L.tcap);
     T.tet = (float4*)take(16 * (size_t)L.tcap);
-    T.fmask = L.fcap - 1;
-    T.vmask = L.vcap - 1;
+    int* cur = (int*)take(4 * (size_t)L.fcap);
+    int* nxt = (int*)take(4 * (size_t)L.fcap);
+    T.hmask = L.hfcap - 1;
+    T.vhmask = L.hvcap - 1;
+    T.fcap = L.fcap;
+    T.vcap = L.vcap;
     T.qcap = L.qcap;
     T.tcap = L.tcap;
-    T.flimit = L.fcap / 2;
-    T.vlimit = (L.vcap * 3) / 4;
+    GeoParams Q = P;
+    Q.spill = nullptr;  // the scratch ring is sized for a whole source
     for (int i = blockIdx.x; i < nList; i += gridDim.x) {
         const int src = list ? list[i] : i;
         if (redoOnly && P.status[src] == 0) continue;  // solved by the LDS variant
-        solveSource(P, T, sh, src);
+        solveSource(Q, T, sh, src, cur, nxt, L.fcap);
     }
 }
 
-// Patch kernel: the reference's submesh face set of every source, one wavefront and ~16 KB of LDS per source (so many
-// sources per CU hide the search's memory latency), written as a compact list (count -1: more than the list holds).
+// Patch kernel: the reference's submesh face set of every source, one wavefront and ~24 KB of LDS per source (so
+// several sources per CU hide the search's memory latency), written as a compact list (count -1: more faces than the
+// list holds).
 struct FaceSet {
-    int32_t* fkey;
-    int fmask, flimit;
+    int32_t* hkey;
+    int16_t* hslot;
+    int32_t* fglob;
+    int hmask, fcap;
 };
 
 __global__ __launch_bounds__(64) void patchKernel(GeoParams P, int32_t* list, int32_t* count, int stride) {
-    __shared__ int32_t fkey[kPatchHashCap];
+    __shared__ int32_t hkey[kPatchHashCap];
+    __shared__ int16_t hslot[kPatchHashCap];
+    __shared__ int32_t fglob[kPatchListCap];
     __shared__ int32_t cur[kPatchListCap], nxt[kPatchListCap];
     __shared__ Shared sh;
-    const int src = blockIdx.x;
-    if (src >= P.nSrc) return;
+    const int src = sourceOfBlock(P);
+    if (src < 0) return;
     const int lane = threadIdx.x;
     const int pi = P.srcIdx ? P.srcIdx[src] : src;
     const int sf = P.pface[pi];
     const int tb = P.toff[src], nT = P.toff[src + 1] - tb;
-    FaceSet T{fkey, kPatchHashCap - 1, min(stride, (kPatchHashCap * 3) / 4)};
-    for (int i = lane; i < kPatchHashCap; i += blockDim.x) fkey[i] = -1;
+    FaceSet T{hkey, hslot, fglob, kPatchHashCap - 1, min(stride, kPatchListCap)};
+    for (int i = lane; i < kPatchHashCap; i += blockDim.x) hkey[i] = -1;
     if (lane == 0) {
         sh.nFaces = sh.overflow = sh.nCur = sh.nNext = sh.goals = 0;
         initSourceFrame(P, sh, pi, sf);
@@ -1006,33 +1152,51 @@ __global__ __launch_bounds__(64) void patchKernel(GeoParams P, int32_t* list, in
         if (lane == 0) count[src] = -1;
         return;
     }
-    if (lane == 0) sh.nCur = 0;
-    __syncthreads();
+    const int n = sh.nFaces;
     int32_t* out = list + (size_t)src * stride;
-    for (int i = lane; i < kPatchHashCap; i += blockDim.x)
-        if (fkey[i] >= 0) out[atomicAdd(&sh.nCur, 1)] = fkey[i];
-    __syncthreads();
-    if (lane == 0) count[src] = sh.nCur;
+    for (int i = lane; i < n; i += blockDim.x) out[i] = fglob[i];
+    if (lane == 0) count[src] = n;
 }
 
 hipError_t launchPatch(const GeoParams& P, int32_t* list, int32_t* count, int stride, hipStream_t stream) {
     if (P.nSrc <= 0 || P.patchRadius == nullptr) return hipSuccess;
     if (stride > kPatchListCap) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(patchKernel, dim3(P.nSrc), dim3(64), 0, stream, P, list, count, stride);
+    hipLaunchKernelGGL(patchKernel, dim3(gridForSources(P)), dim3(64), 0, stream, P, list, count, stride);
     return hipGetLastError();
 }
 
-size_t scratchBytesPerBlock(int fcap, int vcap, int qcap, int tcap) {
+ScratchLayout makeScratchLayout(long faces, long targets) {
+    auto p2 = [](long n) {
+        long p = 1;
+        while (p < n) p <<= 1;
+        return (int)p;
+    };
+    ScratchLayout L{};
+    L.fcap = (int)std::max(faces, 16L);
+    L.vcap = L.fcap + 64;  // a patch has fewer vertices than faces plus its border
+    L.hfcap = p2(2 * (long)L.fcap);
+    L.hvcap = p2(2 * (long)L.vcap);
+    L.qcap = (int)std::max(16 * (long)L.fcap, 4096L);
+    L.tcap = (int)std::max(targets, 1L);
     auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    return 2 * r(4 * (size_t)fcap) + 3 * r(12 * (size_t)fcap) + 3 * r(4 * (size_t)vcap) + 2 * r(8 * (size_t)vcap) + 5 * r(8 * (size_t)qcap) +
-           r(4 * (size_t)qcap) + r(8 * (size_t)qcap) + r(4 * (size_t)tcap) + r(24 * (size_t)tcap) + 2 * r(8 * (size_t)tcap) +
-           r(16 * (size_t)tcap);
+    const size_t F = L.fcap, V = L.vcap, Q = L.qcap, Tn = L.tcap, HF = L.hfcap, HV = L.hvcap;
+    L.bytesPerBlock = 2 * r(4 * HF) + 2 * r(4 * HV)                        // hashes
+                      + 2 * r(4 * F) + 3 * r(12 * F)                       // fglob, fhead; fcv, fnb, vfl
+                      + r(4 * V) + r(V) + r(4 * V) + 2 * r(8 * V)          // vglob, vflg, vfo; vdist, vtan
+                      + 5 * r(8 * Q) + r(4 * Q) + r(8 * Q)                 // ring
+                      + 2 * r(4 * Tn) + r(24 * Tn) + 2 * r(8 * Tn) + r(16 * Tn)  // targets
+                      + 2 * r(4 * F);                                      // frontiers
+    return L;
 }
 
 hipError_t launchGeodesicLds(const GeoParams& P, hipStream_t stream) {
     if (P.nSrc <= 0) return hipSuccess;
-    hipLaunchKernelGGL((geodesicForceLds<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsThreads>), dim3(P.nSrc),
-                       dim3(kLdsThreads), 0, stream, P);
+    if (P.pairEnd)
+        hipLaunchKernelGGL((geodesicForceLds<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsFaceHash, kLdsVertHash, kLdsThreads, true>),
+                           dim3(gridForSources(P)), dim3(kLdsThreads), 0, stream, P);
+    else
+        hipLaunchKernelGGL((geodesicForceLds<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsFaceHash, kLdsVertHash, kLdsThreads, false>),
+                           dim3(gridForSources(P)), dim3(kLdsThreads), 0, stream, P);
     return hipGetLastError();
 }
 
