@@ -83,6 +83,50 @@ def profiled_traffic(config):
     return None
 
 
+def screened_leg(ctx, w, sysm, steps, plain_steps_s):
+    """The learned screen: trained on the GPU on exact pairs of two other bumpy spheres (seeds 13378, 13379),
+    calibrated on two more (13380, 13381), then timed on this workload from its current state. The training systems live in
+    a context of their own, so that this workload's mesh and particles stay in place."""
+    import curvedspacesim_amd as csa
+    from curvedspacesim_amd import screen, workloads
+
+    t0 = time.perf_counter()
+    tctx = csa.Context(0)
+    built = screen.build(tctx, config=3, train_seeds=(13378, 13379), holdout_seeds=(13380, 13381), steps=2000, batch=8192)
+    tctx.close()
+    t_build = time.perf_counter() - t0
+    log(f"screen built in {t_build:.1f} s: threshold {built['threshold']:.3f}, {built['train_pairs']} training pairs")
+    # check on this state: forces of the screened computation, and whether any dropped pair lies within the cutoff
+    sysm.record_pairs(True)
+    F0, E0 = sysm.compute_forces()
+    d = sysm.pair_distances()
+    X = sysm.screen_features()
+    sysm.record_pairs(False)
+    dropped = screen.dropped(ctx, built["params"], built["threshold"], X)
+    sysm.set_screen(built["params"], built["threshold"])
+    F1, E1 = sysm.compute_forces()
+    dF = float(np.max(np.abs(F1 - F0)) / np.abs(F0).max())
+    dE = float(abs(E1 - E0) / abs(E0))
+    match = dF <= 1e-12 and dE <= 1e-12
+    workloads.step(sysm, w, 1)
+    sysm.sync()
+    t0 = time.perf_counter()
+    workloads.step(sysm, w, steps)
+    sysm.sync()
+    el = time.perf_counter() - t0
+    st = sysm.screen_stats()
+    sysm.set_screen(None)
+    v = steps / el
+    return {"value": v, "unit": "steps/s", "speedup_vs_plain": v / plain_steps_s, "pairs_solved_per_step": st["kept"],
+            "candidate_pairs_per_step": st["candidates"], "dropped_fraction": 1 - st["kept"] / max(st["candidates"], 1),
+            "forces_match_plain": match, "max_force_difference_rel": dF, "energy_difference_rel": dE,
+            "dropped_within_cutoff": int(np.sum(dropped & (d < w.range))),
+            "beyond_cutoff_fraction": float(np.mean(d >= w.range)), "threshold": built["threshold"],
+            "training_pairs": built["train_pairs"],
+            "holdout_beyond_cutoff_fraction": built["holdout_beyond"], "holdout_dropped_fraction": built["holdout_drop"],
+            "train_and_calibrate_s": t_build}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +135,7 @@ def main():
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-screen", action="store_true", help="skip the learned-screen leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}): this build measures one GPU's share of the "
@@ -104,6 +149,8 @@ def main():
     log(f"workload: {w.name}")
     ctx = csa.Context(0)
     sysm = workloads.make_system(ctx, w)
+    chosen, tune_ms = sysm.autotune()
+    log(f"autotune: LDS kernel variant {chosen} ({[256, 512][chosen]} lanes per source); ms per force computation {tune_ms}")
     log(f"warmup {args.warmup} steps")
     workloads.step(sysm, w, args.warmup)
     sysm.sync()
@@ -145,7 +192,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": w.name, "N": int(len(w.face)), "faces": int(len(w.T)), "cutoff": w.range,
-                   "integrator": w.integrator, "dt": w.dt, "screen": "plain (no learned screen)"},
+                   "integrator": w.integrator, "dt": w.dt, "screen": "plain (no learned screen)",
+                   "lanes_per_source": [256, 512][chosen]},
+        "autotune_ms_per_force": {"256_lanes": float(tune_ms[0]), "512_lanes": float(tune_ms[1])},
         "pairs_per_step": int(pairs) * forces_per_step,
         "pairs_per_s": pairs * forces_per_step * steps_s,
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kt.items()},
@@ -159,6 +208,8 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args.config),
                      "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_launch},
     }
+    if not args.no_screen and args.config == 3:
+        out["screened"] = screened_leg(ctx, w, sysm, args.steps, steps_s)
     if not args.no_cpu_baseline:
         f, b, v, _ = sysm.get()
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

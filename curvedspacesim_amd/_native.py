@@ -29,6 +29,7 @@ class csg_potential(C.Structure):
 
 HARMONIC, GAUSSIAN, TRUNC_GAUSSIAN, NO_POTENTIAL = 0, 1, 2, 3
 PAIR_UNREACHED, PAIR_BEYOND, PAIR_SENTINEL = 1, 4, 16
+SCREEN_FEATURES, SCREEN_PARAMS = 16, 5313
 
 
 def _path(name):
@@ -73,6 +74,17 @@ def csg():
     L.csg_system_kernel_times.argtypes = [VP, _f64p, _i32p]
     L.csg_system_patch_stats.argtypes = [VP, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"), _i32p]
     L.csg_system_fallback_counts.argtypes = [VP, _i32p]
+    _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.csg_system_autotune.argtypes = [VP, C.POINTER(C.c_int32), _f64p]
+    L.csg_system_set_lds_variant.argtypes = [VP, C.c_int32]
+    L.csg_system_record_pairs.argtypes = [VP, C.c_int32]
+    L.csg_system_pair_distances.argtypes = [VP, _f64p, C.c_int64]
+    L.csg_system_screen_features.argtypes = [VP, _f32p, C.c_int64]
+    L.csg_system_set_screen.argtypes = [VP, VP, C.c_float]
+    L.csg_system_screen_stats.argtypes = [VP, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.csg_screen_forward.argtypes = [VP, _f32p, C.c_int32, _f32p, _f32p]
+    L.csg_screen_gradient.argtypes = [VP, _f32p, C.c_int32, _f32p, _f32p, _f32p, C.POINTER(C.c_float)]
+    L.csg_screen_train.argtypes = [VP, _f32p, C.c_int32, _f32p, _f32p, C.c_int32, C.c_int32, C.c_float, C.c_uint32, VP]
     _csg = L
     return L
 
@@ -142,6 +154,32 @@ class Context:
         if rc != 0:
             raise CsgError(f"csg_ctx_create failed ({rc}): no gfx950 device {device} or HIP runtime error")
         self.h = h
+
+    # --- learned pair screen (csg_screen_*) ---
+    def screen_forward(self, params, X):
+        p = np.ascontiguousarray(params, np.float32)
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, SCREEN_FEATURES)
+        out = np.zeros(len(X), np.float32)
+        self._check(self.L.csg_screen_forward(self.h, p, len(X), X, out), "csg_screen_forward")
+        return out
+
+    def screen_gradient(self, params, X, y):
+        p = np.ascontiguousarray(params, np.float32)
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, SCREEN_FEATURES)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.zeros(SCREEN_PARAMS, np.float32)
+        loss = C.c_float()
+        self._check(self.L.csg_screen_gradient(self.h, p, len(X), X, y, g, C.byref(loss)), "csg_screen_gradient")
+        return g, loss.value
+
+    def screen_train(self, params, X, y, steps, batch=4096, lr=1e-3, seed=1):
+        p = np.array(params, np.float32).copy()
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, SCREEN_FEATURES)
+        y = np.ascontiguousarray(y, np.float32)
+        losses = np.zeros(max(int(steps), 1), np.float32)
+        self._check(self.L.csg_screen_train(self.h, p, len(X), X, y, int(steps), int(batch), float(lr), int(seed),
+                                            losses.ctypes.data), "csg_screen_train")
+        return p, losses[:steps]
 
     def close(self):
         if getattr(self, "h", None):
@@ -240,9 +278,18 @@ class Context:
 
 
 class System:
-    """Device-resident particle system of a Context (csg_system_* entry points)."""
+    """Device-resident particle system of a Context (csg_system_* entry points). A context holds one system: creating
+    another System on the same context replaces it, and the older handle then refuses every call."""
+
+    def __getattribute__(self, name):
+        if not name.startswith("_") and name not in ("ctx", "L", "N"):
+            if object.__getattribute__(self, "_gen") != getattr(object.__getattribute__(self, "ctx"), "_sys_gen", 0):
+                raise CsgError("this System was replaced by a newer System on the same Context")
+        return object.__getattribute__(self, name)
 
     def __init__(self, ctx, face, bary, vel=None, mass=None):
+        ctx._sys_gen = getattr(ctx, "_sys_gen", 0) + 1
+        self._gen = ctx._sys_gen
         self.ctx = ctx
         self.L = ctx.L
         face = np.ascontiguousarray(face, np.int32)
@@ -303,11 +350,48 @@ class System:
         self.ctx._check(self.L.csg_system_profile(self.ctx.h, int(bool(enable))), "csg_system_profile")
 
     def kernel_times(self):
-        ms = np.zeros(5)
-        n = np.zeros(5, np.int32)
+        ms = np.zeros(6)
+        n = np.zeros(6, np.int32)
         self.ctx._check(self.L.csg_system_kernel_times(self.ctx.h, ms, n), "csg_system_kernel_times")
-        names = ["neighbors", "geodesic_lds", "geodesic_hbm", "transport", "patch"]
+        names = ["neighbors", "geodesic_lds", "geodesic_hbm", "transport", "patch", "screen"]
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(names)}
+
+    def autotune(self):
+        """Times each workgroup size of the LDS geodesic kernel and keeps the fastest: (chosen, ms per variant)."""
+        v = C.c_int32()
+        ms = np.zeros(2)
+        self.ctx._check(self.L.csg_system_autotune(self.ctx.h, C.byref(v), ms), "csg_system_autotune")
+        return v.value, ms
+
+    def set_lds_variant(self, variant):
+        self.ctx._check(self.L.csg_system_set_lds_variant(self.ctx.h, int(variant)), "csg_system_set_lds_variant")
+
+    def record_pairs(self, enable=True):
+        self.ctx._check(self.L.csg_system_record_pairs(self.ctx.h, int(bool(enable))), "csg_system_record_pairs")
+
+    def pair_distances(self):
+        off, _ = self.pairs()
+        d = np.zeros(max(int(off[-1]), 1))
+        self.ctx._check(self.L.csg_system_pair_distances(self.ctx.h, d, len(d)), "csg_system_pair_distances")
+        return d[: off[-1]]
+
+    def screen_features(self):
+        off, _ = self.pairs()
+        X = np.zeros((max(int(off[-1]), 1), SCREEN_FEATURES), np.float32)
+        self.ctx._check(self.L.csg_system_screen_features(self.ctx.h, X, len(X)), "csg_system_screen_features")
+        return X[: off[-1]]
+
+    def set_screen(self, params=None, threshold=0.0):
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        if p is not None and p.size != SCREEN_PARAMS:
+            raise CsgError(f"screen parameters must hold {SCREEN_PARAMS} floats")
+        self.ctx._check(self.L.csg_system_set_screen(self.ctx.h, None if p is None else p.ctypes.data, float(threshold)),
+                        "csg_system_set_screen")
+
+    def screen_stats(self):
+        a, b = C.c_int64(), C.c_int64()
+        self.ctx._check(self.L.csg_system_screen_stats(self.ctx.h, C.byref(a), C.byref(b)), "csg_system_screen_stats")
+        return dict(candidates=a.value, kept=b.value)
 
     def fallback_counts(self):
         r = np.zeros(6, np.int32)

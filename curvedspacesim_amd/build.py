@@ -8,7 +8,7 @@ CSRC = os.path.join(PKG, "csrc")
 ROOT = os.path.dirname(PKG)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["csg_api.hip", "csg_system.hip", "geodesic_kernel.hip", "transport_kernel.hip"]
+HIP_SOURCES = ["csg_api.hip", "csg_system.hip", "csg_screen.hip", "geodesic_kernel.hip", "transport_kernel.hip", "screen_kernel.hip"]
 HOST_SOURCES = ["host/meshGenerators.cpp", "host/hostCapi.cpp", "host/noiseSource.cpp", "host/gpuMeshSpace.cpp",
                 "host/neighborStructures.cpp", "host/simpleModel.cpp", "host/forces.cpp", "host/simulation.cpp"]
 # reference-style scenario driver of the C++ host classes, run by the parity tests

@@ -33,6 +33,57 @@ hipError_t download(csg_ctx* c, T* host, const DevBuf& b, size_t n) {
 }
 
 // Face adjacency, per-edge frames, normals, vertex classes, as the reference's Surface_mesh would provide them.
+// Per vertex (mean curvature H, Gaussian curvature K), features of the learned screen: K = (2 pi - angle sum) / A and
+// H from the cotangent Laplacian, Delta x = (1 / 2A) sum_faces cot(theta_k) (x_j - x_i) over the edge (i, j) opposite
+// corner k, as H = -n . Delta x / 2 with the area-weighted vertex normal n; A is a third of the incident face area.
+// Border vertices get (0, 0).
+std::vector<double> vertexCurvatures(int nV, const double* X, int nF, const int32_t* F, const std::vector<double>& angle,
+                                     const std::vector<uint8_t>& flag) {
+    std::vector<double> lap(3 * (size_t)nV, 0.0), vn(3 * (size_t)nV, 0.0), area(nV, 0.0), out(2 * (size_t)nV, 0.0);
+    for (int f = 0; f < nF; ++f) {
+        const double* p[3] = {X + 3 * F[3 * f], X + 3 * F[3 * f + 1], X + 3 * F[3 * f + 2]};
+        double e1[3], e2[3], n[3];
+        for (int a = 0; a < 3; ++a) {
+            e1[a] = p[1][a] - p[0][a];
+            e2[a] = p[2][a] - p[0][a];
+        }
+        n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+        n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+        n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+        const double A = 0.5 * std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        for (int k = 0; k < 3; ++k) {
+            const int vi = F[3 * f + (k + 1) % 3], vj = F[3 * f + (k + 2) % 3];
+            const double* pk = p[k];
+            const double* pi = p[(k + 1) % 3];
+            const double* pj = p[(k + 2) % 3];
+            double u[3], w[3];
+            for (int a = 0; a < 3; ++a) {
+                u[a] = pi[a] - pk[a];
+                w[a] = pj[a] - pk[a];
+            }
+            const double dot = u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
+            const double cr[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+            const double cot = dot / std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+            for (int a = 0; a < 3; ++a) {
+                lap[3 * vi + a] += cot * (pj[a] - pi[a]);
+                lap[3 * vj + a] += cot * (pi[a] - pj[a]);
+            }
+            const int v = F[3 * f + k];
+            area[v] += A / 3.0;
+            for (int a = 0; a < 3; ++a) vn[3 * v + a] += n[a];
+        }
+    }
+    for (int v = 0; v < nV; ++v) {
+        if ((flag[v] & 2) || !(area[v] > 0)) continue;
+        const double nl = std::sqrt(vn[3 * v] * vn[3 * v] + vn[3 * v + 1] * vn[3 * v + 1] + vn[3 * v + 2] * vn[3 * v + 2]);
+        double h = 0;
+        for (int a = 0; a < 3; ++a) h += -(vn[3 * v + a] / nl) * lap[3 * v + a] / (2.0 * area[v]) / 2.0;
+        out[2 * v] = h;
+        out[2 * v + 1] = (2 * M_PI - angle[v]) / area[v];
+    }
+    return out;
+}
+
 csg_status preprocessMesh(csg_ctx* c, int nV, const double* X, int nF, const int32_t* F) {
     if (nV <= 0 || nF <= 0 || !X || !F) return fail(c, CSG_ERR_ARGUMENT, "empty mesh");
     for (long i = 0; i < 3L * nF; ++i)
@@ -110,6 +161,8 @@ csg_status preprocessMesh(csg_ctx* c, int nV, const double* X, int nF, const int
     HIPCHECK(c, upload(c, c->vflag, flag.data(), flag.size()));
     HIPCHECK(c, upload(c, c->vfoff, vfoff.data(), vfoff.size()));
     HIPCHECK(c, upload(c, c->vflist, vflist.data(), vflist.size()));
+    std::vector<double> curv = vertexCurvatures(nV, X, nF, F, angle, flag);
+    HIPCHECK(c, upload(c, c->curv, curv.data(), curv.size()));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     c->nV = nV;
     c->nF = nF;
@@ -153,7 +206,7 @@ csg_status runGeodesics(csg_ctx* c, csg::GeoParams& G, int maxTargets, bool whol
             G.spillCap = csg::kSpillCap;
             HIPCHECK(c, csg::launchPatch(G, c->plist.as<int32_t>(), c->pcount.as<int32_t>(), G.patchStride, c->stream));
         }
-        HIPCHECK(c, csg::launchGeodesicLds(G, c->stream));
+        HIPCHECK(c, csg::launchGeodesicLds(G, 0, c->stream));
         std::vector<int> st(G.nSrc);
         HIPCHECK(c, hipMemcpyAsync(st.data(), G.status, sizeof(int) * G.nSrc, hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(c, hipStreamSynchronize(c->stream));

@@ -50,7 +50,7 @@ struct csg_ctx {
     // mesh
     int nV = 0, nF = 0;
     double area = 0, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    DevBuf vx, fv, fadj, fgeo, fnrm, vflag, vfoff, vflist;
+    DevBuf vx, fv, fadj, fgeo, fnrm, vflag, vfoff, vflist, curv;
     csg::DeviceMesh dm{};
     bool haveMesh = false;
     int submeshing = 0;
@@ -58,13 +58,14 @@ struct csg_ctx {
     // per-call staging
     DevBuf pface, pbary, srcIdx, toff, tidx, prad, pdist, pst, pet, pflags, pforces, penergy, status, list, plist, pcount, spill;
     DevBuf tdisp, tvecs, tstat;
+    DevBuf sX, sY, sOut, sPart, sGrad, sParams, sM, sV;  // learned-screen staging
     DevBuf scratch;
     int scratchBlocks = 0;
     csg::ScratchLayout layout{};
     csg::SystemState* sys = nullptr;
     ~csg_ctx() {
         if (sys) csg::destroySystem(sys);
-        DevBuf* all[] = {&vx, &fv, &fadj, &fgeo, &fnrm, &vflag, &vfoff, &vflist, &pface, &pbary, &srcIdx, &toff, &tidx, &prad,
+        DevBuf* all[] = {&vx, &fv, &fadj, &fgeo, &fnrm, &vflag, &vfoff, &vflist, &curv, &sX, &sY, &sOut, &sPart, &sGrad, &sParams, &sM, &sV, &pface, &pbary, &srcIdx, &toff, &tidx, &prad,
                          &pdist, &pst, &pet, &pflags, &pforces, &penergy, &status, &list, &plist, &pcount, &spill, &tdisp, &tvecs, &tstat, &scratch};
         for (DevBuf* b : all) b->release();
         if (stream) hipStreamDestroy(stream);

@@ -23,12 +23,13 @@
 #include "csg_common.h"
 #include "csg_context.h"
 #include "geodesic_kernel.h"
+#include "screen_kernel.h"
 #include "transport_kernel.h"
 
 namespace csg {
 
 // device counters; CNT_LDS_REASON + b counts sources the LDS variant handed to the HBM variant for OverflowReason bit b
-enum Counter : int { CNT_CAND_OVERFLOW = 0, CNT_UNSOLVED = 1, CNT_WALK_FAIL = 2, CNT_PAIRS = 3, CNT_LDS_REASON = 4, CNT_N = 10 };
+enum Counter : int { CNT_CAND_OVERFLOW = 0, CNT_UNSOLVED = 1, CNT_WALK_FAIL = 2, CNT_PAIRS = 3, CNT_LDS_REASON = 4, CNT_KEPT = 10, CNT_N = 11 };
 
 struct SystemState {
     int N = 0;
@@ -37,6 +38,11 @@ struct SystemState {
     DevBuf cnt, off, toff, cand, maxd, rad;
     DevBuf disp, tstatus, gstatus, partial, counters, ke, chain, pstat, plist, pcount, spill;
     DevBuf scratch;
+    // learned screen: features and logits of the candidate pairs, kept-pair lists; recorded pair distances
+    DevBuf sfeat, slogit, scnt, soff, scand, sparams, pdist;
+    bool screenOn = false, recordPairs = false;
+    int ldsVariant = 0;  // workgroup size of the LDS geodesic kernel, chosen by csg_system_autotune
+    float screenThreshold = 0.f;
     ScratchLayout layout{};
     int scratchBlocks = 0;
     // configuration
@@ -61,7 +67,7 @@ struct SystemState {
     std::vector<Mark> marks;
 };
 
-enum MarkKind : int { MARK_NEIGHBORS = 0, MARK_GEODESIC_LDS = 1, MARK_GEODESIC_HBM = 2, MARK_TRANSPORT = 3, MARK_PATCH = 4, MARK_N = 5 };
+enum MarkKind : int { MARK_NEIGHBORS = 0, MARK_GEODESIC_LDS = 1, MARK_GEODESIC_HBM = 2, MARK_TRANSPORT = 3, MARK_PATCH = 4, MARK_SCREEN = 5, MARK_N = 6 };
 
 // records an event pair around a group of launches on the context stream when profiling is on
 struct MarkScope {
@@ -85,7 +91,8 @@ struct MarkScope {
 void destroySystem(SystemState* s) {
     DevBuf* all[] = {&s->face, &s->bary, &s->vel, &s->force, &s->mass, &s->energy, &s->xyz, &s->key, &s->cellCount, &s->cellStart,
                      &s->cellFill, &s->order, &s->cnt, &s->off, &s->toff, &s->cand, &s->maxd, &s->rad, &s->disp, &s->tstatus,
-                     &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->spill, &s->scratch};
+                     &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->spill, &s->scratch,
+                     &s->sfeat, &s->slogit, &s->scnt, &s->soff, &s->scand, &s->sparams, &s->pdist};
     for (DevBuf* b : all) b->release();
     for (auto& m : s->marks) {
         hipEventDestroy(m.a);
@@ -355,6 +362,31 @@ __global__ void kScaleVel(int N, double* v, const double* ke) {
 
 }  // namespace
 
+// the screen keeps a pair whose logit is at or above the threshold, and every pair closer than the guard distance
+__device__ inline bool screenKeeps(const float* feat, const float* logit, int q, float thr) {
+    return logit[q] >= thr || feat[(size_t)q * kScreenIn] > kScreenGuard;
+}
+
+// pairs of source i that the screen keeps
+__global__ void kScreenCount(int N, const int* toff, const float* feat, const float* logit, float thr, int* cnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int c = 0;
+    for (int q = toff[i]; q < toff[i + 1]; ++q) c += screenKeeps(feat, logit, q, thr) ? 1 : 0;
+    cnt[i] = c;
+}
+
+// the kept candidates of each source, in their original order
+__global__ void kScreenScatter(int N, const int* toff, const int* cand, const float* feat, const float* logit, float thr, const int* soff,
+                               int* scand, int* counters) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int o = soff[i];
+    for (int q = toff[i]; q < toff[i + 1]; ++q)
+        if (screenKeeps(feat, logit, q, thr)) scand[o++] = cand[q];
+    if (i == 0) counters[CNT_KEPT] = soff[N];
+}
+
 // ---- step pieces (all asynchronous on the context stream) ------------------------------------------------------
 
 static csg_status sysNeighbors(csg_ctx* c, SystemState* S) {
@@ -399,6 +431,42 @@ static csg_status sysNeighbors(csg_ctx* c, SystemState* S) {
     return CSG_OK;
 }
 
+// features of every candidate pair of the last neighbour search (the pair count is read on the device)
+static void screenFeatures(csg_ctx* c, SystemState* S) {
+    ScreenFeatureParams F{};
+    F.mesh = c->dm;
+    F.curv = c->curv.as<double>();
+    F.face = S->face.as<int32_t>();
+    F.bary = S->bary.as<double>();
+    F.xyz = S->xyz.as<double>();
+    F.nSrc = S->N;
+    F.toff = S->toff.as<int32_t>();
+    F.tidx = S->cand.as<int32_t>();
+    F.patchCount = c->submeshing ? S->pcount.as<int32_t>() : nullptr;
+    F.cutoff = S->range;
+    F.nPairsDev = S->toff.as<int32_t>() + S->N;
+    F.out = S->sfeat.as<float>();
+    (void)launchScreenFeatures(F, (int)std::min<long>(S->candCap, 0x7fffffff), c->stream);
+}
+
+// screen the candidate pairs and point the solver at the kept lists
+static csg_status sysScreen(csg_ctx* c, SystemState* S, GeoParams& G) {
+    const int N = S->N;
+    hipStream_t st = c->stream;
+    screenFeatures(c, S);
+    HIPCHECK(c, launchScreenForward(S->sparams.as<float>(), S->sfeat.as<float>(), (int)std::min<long>(S->candCap, 0x7fffffff),
+                                    S->toff.as<int32_t>() + N, S->slogit.as<float>(), st));
+    hipLaunchKernelGGL(kScreenCount, dim3(blocksFor(N)), dim3(kBlock), 0, st, N, S->toff.as<int>(), S->sfeat.as<float>(), S->slogit.as<float>(),
+                       S->screenThreshold, S->scnt.as<int>());
+    hipLaunchKernelGGL(kScanExclusive, dim3(1), dim3(1024), 0, st, S->scnt.as<int>(), S->soff.as<int>(), N);
+    hipLaunchKernelGGL(kScreenScatter, dim3(blocksFor(N)), dim3(kBlock), 0, st, N, S->toff.as<int>(), S->cand.as<int>(), S->sfeat.as<float>(),
+                       S->slogit.as<float>(), S->screenThreshold, S->soff.as<int>(), S->scand.as<int>(), S->counters.as<int>());
+    HIPCHECK(c, hipGetLastError());
+    G.toff = S->soff.as<int32_t>();
+    G.tidx = S->scand.as<int32_t>();
+    return CSG_OK;
+}
+
 static csg_status sysForces(csg_ctx* c, SystemState* S) {
     csg_status s = sysNeighbors(c, S);
     if (s) return s;
@@ -420,6 +488,7 @@ static csg_status sysForces(csg_ctx* c, SystemState* S) {
     G.sentinelDist = 2.0 * c->maximumDistance;
     G.rMax = exactTruncation(S->pot, S->rMax, G.useSentinel != 0, G.sentinelDist);
     G.patchStats = S->pstat.as<int>();
+    G.pairDist = S->recordPairs ? S->pdist.as<double>() : nullptr;
     if (S->cellList) {  // sources in cell order, one contiguous chunk per XCD
         G.blockOrder = S->order.as<int32_t>();
         G.blockChunk = (S->N + kXcds - 1) / kXcds;
@@ -434,9 +503,14 @@ static csg_status sysForces(csg_ctx* c, SystemState* S) {
             MarkScope mark(S, c->stream, MARK_PATCH);
             HIPCHECK(c, launchPatch(G, S->plist.as<int32_t>(), S->pcount.as<int32_t>(), kPatchListCap, c->stream));
         }
+        if (S->screenOn) {  // the solver takes only the pairs the screen keeps; patches keep every candidate's face
+            MarkScope mark(S, c->stream, MARK_SCREEN);
+            csg_status s2 = sysScreen(c, S, G);
+            if (s2) return s2;
+        }
         {
             MarkScope mark(S, c->stream, MARK_GEODESIC_LDS);
-            HIPCHECK(c, launchGeodesicLds(G, c->stream));
+            HIPCHECK(c, launchGeodesicLds(G, S->ldsVariant, c->stream));
         }
         hipLaunchKernelGGL(kCountReasons, dim3(blocksFor(S->N)), dim3(kBlock), 0, c->stream, S->N, S->gstatus.as<int>(), S->counters.as<int>());
         MarkScope mark(S, c->stream, MARK_GEODESIC_HBM);
@@ -777,6 +851,128 @@ csg_status csg_system_fallback_counts(csg_ctx* c, int32_t* reasons) {
     int cnt[CNT_N];
     HIPCHECK(c, hipMemcpy(cnt, S->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
     for (int b = 0; b < 6; ++b) reasons[b] = cnt[CNT_LDS_REASON + b];
+    return CSG_OK;
+}
+
+csg_status csg_system_autotune(csg_ctx* c, int32_t* chosen, double* ms) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S || !S->configured) return fail(c, CSG_ERR_STATE, "csg_system_init and csg_system_configure first");
+    HIPCHECK(c, hipSetDevice(c->device));
+    const bool prof = S->profiling;
+    S->profiling = false;
+    hipEvent_t a, b;
+    HIPCHECK(c, hipEventCreate(&a));
+    HIPCHECK(c, hipEventCreate(&b));
+    int best = 0;
+    double bestMs = HUGE_VAL;
+    for (int v = 0; v < kLdsVariants; ++v) {
+        S->ldsVariant = v;
+        csg_status s = sysForces(c, S);  // untimed: first use of the variant
+        if (s) return s;
+        HIPCHECK(c, hipEventRecord(a, c->stream));
+        for (int r = 0; r < 2; ++r) {
+            s = sysForces(c, S);
+            if (s) return s;
+        }
+        HIPCHECK(c, hipEventRecord(b, c->stream));
+        HIPCHECK(c, hipEventSynchronize(b));
+        float t = 0;
+        HIPCHECK(c, hipEventElapsedTime(&t, a, b));
+        if (ms) ms[v] = t / 2;
+        if (t < bestMs) {
+            bestMs = t;
+            best = v;
+        }
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    S->ldsVariant = best;
+    S->profiling = prof;
+    if (chosen) *chosen = best;
+    return CSG_OK;
+}
+
+csg_status csg_system_set_lds_variant(csg_ctx* c, int32_t variant) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
+    if (variant < 0 || variant >= kLdsVariants) return fail(c, CSG_ERR_ARGUMENT, "unknown LDS kernel variant");
+    S->ldsVariant = variant;
+    return CSG_OK;
+}
+
+csg_status csg_system_record_pairs(csg_ctx* c, int32_t enable) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S || !S->configured) return fail(c, CSG_ERR_STATE, "csg_system_init and csg_system_configure first");
+    HIPCHECK(c, hipSetDevice(c->device));
+    if (enable) HIPCHECK(c, S->pdist.ensure(8 * (size_t)std::max<long>(S->candCap, 1)));
+    S->recordPairs = enable != 0;
+    return CSG_OK;
+}
+
+csg_status csg_system_pair_distances(csg_ctx* c, double* dist, int64_t cap) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S || !S->recordPairs) return fail(c, CSG_ERR_STATE, "csg_system_record_pairs(ctx, 1) before the force computation");
+    if (S->screenOn) return fail(c, CSG_ERR_STATE, "pair distances are recorded for the unscreened lists only");
+    int n = 0;
+    HIPCHECK(c, hipMemcpy(&n, S->toff.as<int>() + S->N, 4, hipMemcpyDeviceToHost));
+    if (n > cap) return fail(c, CSG_ERR_CAPACITY, "distance buffer too small");
+    HIPCHECK(c, hipMemcpy(dist, S->pdist.p, 8 * (size_t)n, hipMemcpyDeviceToHost));
+    return CSG_OK;
+}
+
+csg_status csg_system_screen_features(csg_ctx* c, float* features, int64_t cap) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S || !S->configured) return fail(c, CSG_ERR_STATE, "csg_system_init and csg_system_configure first");
+    if (!S->cellList) return fail(c, CSG_ERR_STATE, "the screen works on cell-list candidates");
+    HIPCHECK(c, hipSetDevice(c->device));
+    HIPCHECK(c, S->sfeat.ensure(sizeof(float) * kScreenIn * (size_t)std::max<long>(S->candCap, 1)));
+    int n = 0;
+    HIPCHECK(c, hipMemcpy(&n, S->toff.as<int>() + S->N, 4, hipMemcpyDeviceToHost));
+    if (n > cap) return fail(c, CSG_ERR_CAPACITY, "feature buffer too small");
+    screenFeatures(c, S);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipMemcpyAsync(features, S->sfeat.p, sizeof(float) * kScreenIn * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return CSG_OK;
+}
+
+csg_status csg_system_set_screen(csg_ctx* c, const float* params, float threshold) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S || !S->configured) return fail(c, CSG_ERR_STATE, "csg_system_init and csg_system_configure first");
+    if (params && (!S->cellList || !c->submeshing)) return fail(c, CSG_ERR_STATE, "the screen needs cell-list candidates and submeshing");
+    HIPCHECK(c, hipSetDevice(c->device));
+    if (!params) {
+        S->screenOn = false;
+        return CSG_OK;
+    }
+    const size_t cap = (size_t)std::max<long>(S->candCap, 1);
+    HIPCHECK(c, S->sfeat.ensure(sizeof(float) * kScreenIn * cap));
+    HIPCHECK(c, S->slogit.ensure(sizeof(float) * cap));
+    HIPCHECK(c, S->scand.ensure(4 * cap));
+    HIPCHECK(c, S->scnt.ensure(4 * ((size_t)S->N + 1)));
+    HIPCHECK(c, S->soff.ensure(4 * ((size_t)S->N + 1)));
+    HIPCHECK(c, S->sparams.ensure(sizeof(float) * kScreenParams));
+    HIPCHECK(c, hipMemcpyAsync(S->sparams.p, params, sizeof(float) * kScreenParams, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    S->screenThreshold = threshold;
+    S->screenOn = true;
+    return CSG_OK;
+}
+
+csg_status csg_system_screen_stats(csg_ctx* c, int64_t* candidates, int64_t* kept) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
+    int cnt[CNT_N];
+    HIPCHECK(c, hipMemcpy(cnt, S->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
+    if (candidates) *candidates = cnt[CNT_PAIRS];
+    if (kept) *kept = S->screenOn ? cnt[CNT_KEPT] : cnt[CNT_PAIRS];
     return CSG_OK;
 }
 

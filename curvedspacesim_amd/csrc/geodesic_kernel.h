@@ -15,7 +15,6 @@ constexpr int kLdsQueueCap = 768;
 constexpr int kLdsTargetCap = 256;
 constexpr int kLdsFaceHash = 1024;
 constexpr int kLdsVertHash = 1024;
-constexpr int kLdsThreads = 256;
 // work items beyond the ring wait in a per-source HBM stack of this many records
 constexpr int kSpillCap = 1024;
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X, each with its own L2
@@ -80,7 +79,9 @@ struct ScratchLayout {
 ScratchLayout makeScratchLayout(long faces, long targets);
 // face sets of every source (requires P.patchRadius); stride <= kPatchListCap
 hipError_t launchPatch(const GeoParams& P, int32_t* list, int32_t* count, int stride, hipStream_t stream);
-hipError_t launchGeodesicLds(const GeoParams& P, hipStream_t stream);
+// LDS variants: 0 = 256 lanes (4 wavefronts) per source, 1 = 512 lanes (8 wavefronts); both run two workgroups per CU
+constexpr int kLdsVariants = 2;
+hipError_t launchGeodesicLds(const GeoParams& P, int variant, hipStream_t stream);
 hipError_t launchGeodesicScratch(const GeoParams& P, const ScratchLayout& L, const int32_t* list, int nList, int blocks,
                                  hipStream_t stream);
 // scratch variant over every source whose P.status is non-zero (left unsolved by the LDS variant); no host round trip

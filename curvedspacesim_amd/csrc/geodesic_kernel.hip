@@ -477,6 +477,14 @@ __device__ inline double wsum(double x) {
     return x;
 }
 
+// Workgroup barrier that orders LDS accesses only: outstanding global loads and stores stay in flight across it
+// (a full barrier would wait for them). Global data written before it is not guaranteed visible after it.
+__device__ inline void ldsBarrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // The overflow bits as every wave of the workgroup sees them: read between two barriers, so that no wave can set a
 // bit (in the phase that follows) before the slowest wave has read them, and all waves take the same branch.
 __device__ inline bool uniformOverflow(const Shared& sh) {
@@ -722,13 +730,14 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
         const int t = sh.tail;
         const int sp = sh.spillTop;
         const bool ovf = sh.overflow != 0;
-        __syncthreads();  // every wave has read the tail and the flags before any wave appends or sets one
+        ldsBarrier();  // every wave has read the tail and the flags before any wave appends or sets one
         if (ovf || (h >= t && sp == 0)) break;
         if (step >= kMaxSteps) {  // every source ends: an unfinished one is reported like a capacity overflow
             if (lane == 0) atomicOr(&sh.overflow, OVF_STEPS);
             break;
         }
         if (h >= t) {  // the ring is empty: the newest spilled records (at most a ring's worth) move back into it
+            __syncthreads();  // a full barrier: every wave's spill stores are visible before they are read
             const int m = min(sp, T.qcap);
             for (int i = lane; i < m; i += blockDim.x) {
                 const SpillRecord r = spill[sp - m + i];
@@ -750,7 +759,6 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
             continue;
         }
         const int nb = min((int)blockDim.x, t - h);
-        C.qBase = h;
         const bool mine = lane < nb;
         const int q = (h + lane) % T.qcap;
         int fk = -1, kind = 0, g = -1, fs = 0, k = 0;  // kind: 0 window, 1 vertex event, 2 vertex-face item
@@ -784,7 +792,9 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
                 y2 = geo[2];
             }
         }
-        // no barrier here: appends of this step land at positions >= t, never on the records [h, h + nb) just read
+        // once every lane holds its record, the ring slots [h, h + nb) are free for this step's appends
+        ldsBarrier();
+        C.qBase = h + nb;
         VCand cP0 = noCand(), cP1 = noCand(), cP2 = noCand();
         bool live = false;
         if (mine) {
@@ -805,7 +815,7 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
                 }
             }
         }
-        __syncthreads();
+        ldsBarrier();
         if (live) {
             if (kind == 0 && T.fhead[fs] >= 0)
                 windowTargets<true, TT>(C, fs, g, k, L, x2, y2, b0, b1, sx, sy, sg, fk & 1, rot);
@@ -813,7 +823,7 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
             finishVertexCand(C, cP1);
             finishVertexCand(C, cP2);
         }
-        __syncthreads();
+        ldsBarrier();
         h += nb;
         if (lane == 0) {
             sh.nSteps += 1;
@@ -1189,14 +1199,22 @@ ScratchLayout makeScratchLayout(long faces, long targets) {
     return L;
 }
 
-hipError_t launchGeodesicLds(const GeoParams& P, hipStream_t stream) {
-    if (P.nSrc <= 0) return hipSuccess;
+template <int NT>
+static void launchLdsVariant(const GeoParams& P, hipStream_t stream) {
     if (P.pairEnd)
-        hipLaunchKernelGGL((geodesicForceLds<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsFaceHash, kLdsVertHash, kLdsThreads, true>),
-                           dim3(gridForSources(P)), dim3(kLdsThreads), 0, stream, P);
+        hipLaunchKernelGGL((geodesicForceLds<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsFaceHash, kLdsVertHash, NT, true>),
+                           dim3(gridForSources(P)), dim3(NT), 0, stream, P);
     else
-        hipLaunchKernelGGL((geodesicForceLds<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsFaceHash, kLdsVertHash, kLdsThreads, false>),
-                           dim3(gridForSources(P)), dim3(kLdsThreads), 0, stream, P);
+        hipLaunchKernelGGL((geodesicForceLds<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsFaceHash, kLdsVertHash, NT, false>),
+                           dim3(gridForSources(P)), dim3(NT), 0, stream, P);
+}
+
+hipError_t launchGeodesicLds(const GeoParams& P, int variant, hipStream_t stream) {
+    if (P.nSrc <= 0) return hipSuccess;
+    if (variant == 1)
+        launchLdsVariant<512>(P, stream);
+    else
+        launchLdsVariant<256>(P, stream);
     return hipGetLastError();
 }
 

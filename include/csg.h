@@ -131,9 +131,9 @@ csg_status csg_system_get(csg_ctx* ctx, int32_t* face, double* bary, double* vel
 csg_status csg_system_status(csg_ctx* ctx, int64_t* pairsLastForce, int32_t* candOverflow, int32_t* unsolved, int32_t* walkFailures,
                              double* kineticEnergy);
 /* event timing of the step's launches (no reference counterpart; replaces the wall-clock profiler of
- * src/utility/profiler.h:16-59): when enabled, HIP events on the context stream bracket each launch group; ms[5] and
- * launches[5] receive the totals since the last query for {neighbour search, geodesic LDS kernel, geodesic HBM
- * kernel, walk kernel, patch kernel} */
+ * src/utility/profiler.h:16-59): when enabled, HIP events on the context stream bracket each launch group; ms[6] and
+ * launches[6] receive the totals since the last query for {neighbour search, geodesic LDS kernel, geodesic HBM
+ * kernel, walk kernel, patch kernel, learned screen} */
 csg_status csg_system_profile(csg_ctx* ctx, int32_t enable);
 csg_status csg_system_kernel_times(csg_ctx* ctx, double* ms, int32_t* launches);
 /* work statistics of the last force computation: sums[7] and maxima[7] over sources of {patch faces, patch vertices,
@@ -143,8 +143,48 @@ csg_status csg_system_patch_stats(csg_ctx* ctx, int64_t* sums, int32_t* maxima);
 /* sources handed from the LDS kernel to the HBM kernel since csg_system_init, by reason: reasons[6] = {patch faces,
  * patch vertices, live windows, targets, step limit, search frontier} */
 csg_status csg_system_fallback_counts(csg_ctx* ctx, int32_t* reasons);
-/* candidate lists of the last force computation (offsets N+1, indices up to cap) */
+/* autotuner (no reference counterpart): times the force computation of the current state with each workgroup size of
+ * the LDS geodesic kernel (variant 0: 256 lanes, 1: 512 lanes per source) and keeps the fastest for later steps;
+ * chosen and ms[2] (per force computation) nullable. The forces of the state are recomputed, the particles are not
+ * moved. */
+csg_status csg_system_autotune(csg_ctx* ctx, int32_t* chosen, double* ms);
+/* sets the LDS kernel variant directly (0 or 1, as above) */
+csg_status csg_system_set_lds_variant(csg_ctx* ctx, int32_t variant);
+/* candidate lists of the last force computation (offsets N+1, indices up to cap), before any screening */
 csg_status csg_system_pairs(csg_ctx* ctx, int32_t* offsets, int32_t* idx, int64_t cap);
+/* keep the geodesic distance of every candidate pair of later force computations (no reference counterpart; the
+ * training data of the learned screen); csg_system_pair_distances copies those of the last one (screen off) */
+csg_status csg_system_record_pairs(csg_ctx* ctx, int32_t enable);
+csg_status csg_system_pair_distances(csg_ctx* ctx, double* dist, int64_t cap);
+
+/* --- learned pair screen (no reference counterpart) ------------------------------------------------------------------
+ * An MLP 16 -> 64 -> 64 -> 1 with ReLU scores how likely a candidate pair lies within the cutoff; it runs in BF16 on the
+ * matrix cores with FP32 accumulation. Parameters: CSG_SCREEN_PARAMS floats, W1[16][64], b1[64], W2[64][64], b2[64],
+ * W3[64], b3 (W1[i][j] multiplies feature i into hidden unit j). Features of a pair (i, j), with d the Euclidean
+ * separation x_j - x_i, rc the potential's range, n the face normals, H and K the mean and Gaussian curvature
+ * interpolated from the vertices, and h_i = d.n_i, h_j = d.n_j:
+ *   100 (1 - |d|/rc), n_i.n_j, n_i (3), n_j (3), H_i rc, K_i rc^2, H_j rc, K_j rc^2, patch faces of i / 1000,
+ *   100 h_i^2 / (6 |d| rc), 100 h_j^2 / (6 |d| rc), 10 h_i / |d|
+ * (the last three estimate the arc-over-chord excess and the chord's slope). */
+#define CSG_SCREEN_FEATURES 16
+#define CSG_SCREEN_PARAMS 5313
+/* logits of n feature rows */
+csg_status csg_screen_forward(csg_ctx* ctx, const float* params, int32_t n, const float* features, float* logits);
+/* gradient (CSG_SCREEN_PARAMS floats) and value of the mean binary cross-entropy with logits over n labelled rows */
+csg_status csg_screen_gradient(csg_ctx* ctx, const float* params, int32_t n, const float* features, const float* labels, float* grad,
+                               float* loss);
+/* steps of Adam (beta 0.9, 0.999) on minibatches of one seeded shuffle of the rows, on the device; params in/out;
+ * losses (nullable) receives each step's minibatch loss */
+csg_status csg_screen_train(csg_ctx* ctx, float* params, int32_t n, const float* features, const float* labels, int32_t steps,
+                            int32_t batch, float lr, uint32_t seed, float* losses);
+/* features of the candidate pairs of the last force computation (cell list and submeshing required) */
+csg_status csg_system_screen_features(csg_ctx* ctx, float* features, int64_t cap);
+/* screen later force computations: pairs whose logit is below threshold and whose Euclidean separation is at least
+ * 0.985 rc are not solved (params NULL: screen off); closer pairs are always kept. Patches are still built from every
+ * candidate, so every kept pair gets the distance of the plain computation. */
+csg_status csg_system_set_screen(csg_ctx* ctx, const float* params, float threshold);
+/* candidate pairs and kept pairs of the last force computation */
+csg_status csg_system_screen_stats(csg_ctx* ctx, int64_t* candidates, int64_t* kept);
 
 #ifdef __cplusplus
 }
