@@ -25,7 +25,16 @@ def log(*a):
 
 
 def cpu_baseline(w, face, bary, vel, threads, budget_s=30.0):
-    """The FP64 CPU restatement of the reference path (oracle, reference Release flags), timed on this host."""
+    """The FP64 CPU restatement of the reference path (oracle, reference Release flags), timed on this host with
+    `threads` OpenMP threads, and on one thread (the reference's own execution model) over a smaller sample."""
+    r = _cpu_time(w, face, bary, vel, threads, budget_s)
+    if threads > 1:
+        one = _cpu_time(w, face, bary, vel, 1, budget_s / 3)
+        r["single_thread"] = {"value": one["value"], "unit": one["unit"], "cores": 1, "sample": one["sample"]}
+    return r
+
+
+def _cpu_time(w, face, bary, vel, threads, budget_s):
     from oracle import pyoracle
 
     om = pyoracle.OracleMesh(w.X, w.T, fast=True)

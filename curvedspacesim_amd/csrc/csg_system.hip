@@ -34,7 +34,7 @@ enum Counter : int { CNT_CAND_OVERFLOW = 0, CNT_UNSOLVED = 1, CNT_WALK_FAIL = 2,
 struct SystemState {
     int N = 0;
     DevBuf face, bary, vel, force, mass, energy;
-    DevBuf xyz, key, cellCount, cellStart, cellFill, order;
+    DevBuf xyz, sxyz, key, cellCount, cellStart, cellFill, order;
     DevBuf cnt, off, toff, cand, maxd, rad;
     DevBuf disp, tstatus, gstatus, partial, counters, ke, chain, pstat, plist, pcount, spill;
     DevBuf scratch;
@@ -89,7 +89,7 @@ struct MarkScope {
 };
 
 void destroySystem(SystemState* s) {
-    DevBuf* all[] = {&s->face, &s->bary, &s->vel, &s->force, &s->mass, &s->energy, &s->xyz, &s->key, &s->cellCount, &s->cellStart,
+    DevBuf* all[] = {&s->face, &s->bary, &s->vel, &s->force, &s->mass, &s->energy, &s->xyz, &s->sxyz, &s->key, &s->cellCount, &s->cellStart,
                      &s->cellFill, &s->order, &s->cnt, &s->off, &s->toff, &s->cand, &s->maxd, &s->rad, &s->disp, &s->tstatus,
                      &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->spill, &s->scratch,
                      &s->sfeat, &s->slogit, &s->scnt, &s->soff, &s->scand, &s->sparams, &s->pdist};
@@ -188,11 +188,15 @@ __global__ void kSortCells(int nCells, const int* cellStart, int* order) {
 }
 
 template <bool kFill>
-__global__ void kCandidates(Grid g, int N, const double* xyz, const int* key, const int* cellStart, const int* order, double r2, int* cnt,
+// Candidates of every particle in the reference's order (stencil cells x, then y, then z; ascending index within a
+// cell). Thread t takes the t-th particle in cell order, so a wavefront's particles share stencil cells, and the
+// positions are read from their cell-ordered copy sxyz, contiguous within a cell.
+__global__ void kCandidates(Grid g, int N, const double* sxyz, const int* key, const int* cellStart, const int* order, double r2, int* cnt,
                             const int* off, long cap, int* cand, double* maxd, double* rad, double maxDist, int* counters) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const double* p = xyz + 3 * i;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int i = order[t];
+    const double p[3] = {sxyz[3 * t], sxyz[3 * t + 1], sxyz[3 * t + 2]};
     int c = key[i];
     int cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / (g.n[0] * g.n[1]);
     int k = 0;
@@ -203,12 +207,11 @@ __global__ void kCandidates(Grid g, int N, const double* xyz, const int* key, co
             for (int z = max(0, cz - 1); z <= min(g.n[2] - 1, cz + 1); ++z) {
                 int cc = x + g.n[0] * (y + g.n[1] * z);
                 for (int q = cellStart[cc]; q < cellStart[cc + 1]; ++q) {
-                    int j = order[q];
-                    if (j == i) continue;
-                    double dx = p[0] - xyz[3 * j], dy = p[1] - xyz[3 * j + 1], dz = p[2] - xyz[3 * j + 2];
+                    if (q == t) continue;
+                    double dx = p[0] - sxyz[3 * q], dy = p[1] - sxyz[3 * q + 1], dz = p[2] - sxyz[3 * q + 2];
                     double d2 = dx * dx + dy * dy + dz * dz;
                     if (d2 < r2) {
-                        if (kFill && base + k < cap) cand[base + k] = j;
+                        if (kFill && base + k < cap) cand[base + k] = order[q];
                         ++k;
                         if (d2 > m2) m2 = d2;
                     }
@@ -222,6 +225,14 @@ __global__ void kCandidates(Grid g, int N, const double* xyz, const int* key, co
         maxd[i] = md;
         rad[i] = md < maxDist ? md : maxDist;
     }
+}
+
+// positions in cell order
+__global__ void kGatherPositions(int N, const int* order, const double* xyz, double* sxyz) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int i = order[t];
+    for (int a = 0; a < 3; ++a) sxyz[3 * t + a] = xyz[3 * i + a];
 }
 
 // all-to-all candidates: every other particle by increasing index
@@ -417,12 +428,14 @@ static csg_status sysNeighbors(csg_ctx* c, SystemState* S) {
     hipLaunchKernelGGL(kScatter, dim3(blocksFor(N)), dim3(kBlock), 0, st, N, S->key.as<int>(), S->cellStart.as<int>(), S->cellFill.as<int>(),
                        S->order.as<int>());
     hipLaunchKernelGGL(kSortCells, dim3(blocksFor(S->nCells)), dim3(kBlock), 0, st, S->nCells, S->cellStart.as<int>(), S->order.as<int>());
+    hipLaunchKernelGGL(kGatherPositions, dim3(blocksFor(N)), dim3(kBlock), 0, st, N, S->order.as<int>(), S->xyz.as<double>(),
+                       S->sxyz.as<double>());
     double r2 = S->range * S->range;
-    hipLaunchKernelGGL((kCandidates<false>), dim3(blocksFor(N)), dim3(kBlock), 0, st, g, N, S->xyz.as<double>(), S->key.as<int>(),
+    hipLaunchKernelGGL((kCandidates<false>), dim3(blocksFor(N)), dim3(kBlock), 0, st, g, N, S->sxyz.as<double>(), S->key.as<int>(),
                        S->cellStart.as<int>(), S->order.as<int>(), r2, S->cnt.as<int>(), (const int*)nullptr, S->candCap, (int*)nullptr,
                        (double*)nullptr, (double*)nullptr, c->maximumDistance, S->counters.as<int>());
     hipLaunchKernelGGL(kScanExclusive, dim3(1), dim3(1024), 0, st, S->cnt.as<int>(), S->off.as<int>(), N);
-    hipLaunchKernelGGL((kCandidates<true>), dim3(blocksFor(N)), dim3(kBlock), 0, st, g, N, S->xyz.as<double>(), S->key.as<int>(),
+    hipLaunchKernelGGL((kCandidates<true>), dim3(blocksFor(N)), dim3(kBlock), 0, st, g, N, S->sxyz.as<double>(), S->key.as<int>(),
                        S->cellStart.as<int>(), S->order.as<int>(), r2, S->cnt.as<int>(), S->off.as<int>(), S->candCap, S->cand.as<int>(),
                        S->maxd.as<double>(), S->rad.as<double>(), c->maximumDistance, S->counters.as<int>());
     hipLaunchKernelGGL(kClampOffsets, dim3(blocksFor(N + 1)), dim3(kBlock), 0, st, N, S->off.as<int>(), S->candCap, S->toff.as<int>(),
@@ -581,6 +594,7 @@ csg_status csg_system_init(csg_ctx* c, int32_t N, const int32_t* face, const dou
     HIPCHECK(c, hipMemcpyAsync(S->mass.p, mass ? mass : ones.data(), 8 * (size_t)N, hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, S->energy.ensure(8 * (size_t)N));
     HIPCHECK(c, S->xyz.ensure(24 * (size_t)N));
+    HIPCHECK(c, S->sxyz.ensure(24 * (size_t)N));
     HIPCHECK(c, S->key.ensure(4 * (size_t)N));
     HIPCHECK(c, S->order.ensure(4 * (size_t)N));
     HIPCHECK(c, S->cnt.ensure(4 * ((size_t)N + 1)));
