@@ -77,6 +77,7 @@ def csg():
     _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     L.csg_system_autotune.argtypes = [VP, C.POINTER(C.c_int32), _f64p]
     L.csg_system_set_lds_variant.argtypes = [VP, C.c_int32]
+    L.csg_system_use_graph.argtypes = [VP, C.c_int32]
     L.csg_system_record_pairs.argtypes = [VP, C.c_int32]
     L.csg_system_pair_distances.argtypes = [VP, _f64p, C.c_int64]
     L.csg_system_screen_features.argtypes = [VP, _f32p, C.c_int64]
@@ -362,6 +363,9 @@ class System:
         ms = np.zeros(2)
         self.ctx._check(self.L.csg_system_autotune(self.ctx.h, C.byref(v), ms), "csg_system_autotune")
         return v.value, ms
+
+    def use_graph(self, enable=True):
+        self.ctx._check(self.L.csg_system_use_graph(self.ctx.h, int(bool(enable))), "csg_system_use_graph")
 
     def set_lds_variant(self, variant):
         self.ctx._check(self.L.csg_system_set_lds_variant(self.ctx.h, int(variant)), "csg_system_set_lds_variant")

@@ -42,6 +42,10 @@ struct SystemState {
     DevBuf sfeat, slogit, scnt, soff, scand, sparams, pdist;
     bool screenOn = false, recordPairs = false;
     int ldsVariant = 0;  // workgroup size of the LDS geodesic kernel, chosen by csg_system_autotune
+    // one MD step captured as a HIP graph (replayed per step), and what it was captured for
+    bool useGraph = false;
+    hipGraphExec_t graph = nullptr;
+    double graphKey[6] = {0, 0, 0, 0, 0, 0};
     float screenThreshold = 0.f;
     ScratchLayout layout{};
     int scratchBlocks = 0;
@@ -89,6 +93,7 @@ struct MarkScope {
 };
 
 void destroySystem(SystemState* s) {
+    if (s->graph) hipGraphExecDestroy(s->graph);
     DevBuf* all[] = {&s->face, &s->bary, &s->vel, &s->force, &s->mass, &s->energy, &s->xyz, &s->sxyz, &s->key, &s->cellCount, &s->cellStart,
                      &s->cellFill, &s->order, &s->cnt, &s->off, &s->toff, &s->cand, &s->maxd, &s->rad, &s->disp, &s->tstatus,
                      &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->spill, &s->scratch,
@@ -567,6 +572,83 @@ using namespace csg;
 
 static SystemState* sysOf(csg_ctx* c) { return c ? c->sys : nullptr; }
 
+// velocityVerletNVE::performUpdate (velocityVerletNVE.cpp:3-29): displacement and half kick, move, forces, half kick
+static csg_status stepNve(csg_ctx* c, SystemState* S, double dt) {
+    const int N = S->N;
+    hipLaunchKernelGGL(kVVFirst, dim3(blocksFor(N)), dim3(kBlock), 0, c->stream, N, dt, S->vel.as<double>(), S->force.as<double>(),
+                       S->disp.as<double>());
+    csg_status s = sysMove(c, S);
+    if (s) return s;
+    s = sysForces(c, S);
+    if (s) return s;
+    hipLaunchKernelGGL(kVVKick, dim3(blocksFor(N)), dim3(kBlock), 0, c->stream, N, dt, S->vel.as<double>(), S->force.as<double>());
+    HIPCHECK(c, hipGetLastError());
+    return CSG_OK;
+}
+
+// noseHooverNVT::performUpdate (noseHooverNVT.cpp:42-139): chain half step, half move, forces, kick, half move, chain
+static csg_status stepNvt(csg_ctx* c, SystemState* S, double dt, double T, int M) {
+    const int N = S->N;
+    const double dt2 = 0.5 * dt;
+    const int nb = blocksFor(N);
+    hipLaunchKernelGGL(kNHChain, dim3(1), dim3(1), 0, c->stream, S->chain.as<double>(), S->ke.as<double>(), M, T, dt);
+    hipLaunchKernelGGL(kScaleVel, dim3(blocksFor(3 * N)), dim3(kBlock), 0, c->stream, N, S->vel.as<double>(), S->ke.as<double>());
+    hipLaunchKernelGGL(kScaledCopy, dim3(blocksFor(3 * N)), dim3(kBlock), 0, c->stream, N, dt2, S->vel.as<double>(), S->disp.as<double>());
+    csg_status s = sysMove(c, S);
+    if (s) return s;
+    s = sysForces(c, S);
+    if (s) return s;
+    hipLaunchKernelGGL(kNHKick, dim3(nb), dim3(kBlock), 0, c->stream, N, dt, dt2, S->vel.as<double>(), S->force.as<double>(),
+                       S->mass.as<double>(), S->disp.as<double>(), S->partial.as<double>());
+    hipLaunchKernelGGL(kSumPartials, dim3(1), dim3(1), 0, c->stream, nb, S->partial.as<double>(), S->ke.as<double>());
+    s = sysMove(c, S);
+    if (s) return s;
+    hipLaunchKernelGGL(kNHChain, dim3(1), dim3(1), 0, c->stream, S->chain.as<double>(), S->ke.as<double>(), M, T, dt);
+    hipLaunchKernelGGL(kScaleVel, dim3(blocksFor(3 * N)), dim3(kBlock), 0, c->stream, N, S->vel.as<double>(), S->ke.as<double>());
+    HIPCHECK(c, hipGetLastError());
+    return CSG_OK;
+}
+
+static void dropGraph(SystemState* S) {
+    if (S->graph) hipGraphExecDestroy(S->graph);
+    S->graph = nullptr;
+}
+
+// nSteps of `step`: launched directly, or, in graph mode (and without event timing, whose per-launch events are not
+// captured), as replays of one step captured into a HIP graph for these parameters. The step launches only kernels
+// (no memset or copy nodes) with arguments fixed for given parameters, so every replay repeats the same work.
+template <class Step>
+static csg_status runSteps(csg_ctx* c, SystemState* S, const double key[6], int nSteps, Step step) {
+    if (!S->useGraph || S->profiling || nSteps < 1) {
+        for (int i = 0; i < nSteps; ++i) {
+            csg_status s = step();
+            if (s) return s;
+        }
+        return CSG_OK;
+    }
+    if (!S->graph || !std::equal(key, key + 6, S->graphKey)) {
+        dropGraph(S);
+        HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        csg_status s = step();
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (s) {
+            if (g) hipGraphDestroy(g);
+            return s;
+        }
+        if (e != hipSuccess) return fail(c, CSG_ERR_DEVICE, std::string("step capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&S->graph, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            S->graph = nullptr;
+            return fail(c, CSG_ERR_DEVICE, std::string("step graph: ") + hipGetErrorString(e));
+        }
+        std::copy(key, key + 6, S->graphKey);
+    }
+    for (int i = 0; i < nSteps; ++i) HIPCHECK(c, hipGraphLaunch(S->graph, c->stream));
+    return CSG_OK;
+}
+
 extern "C" {
 
 csg_status csg_system_init(csg_ctx* c, int32_t N, const int32_t* face, const double* bary, const double* vel, const double* mass) {
@@ -623,6 +705,7 @@ csg_status csg_system_configure(csg_ctx* c, const csg_potential* pot, double ran
     if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
     if (!(range > 0)) return fail(c, CSG_ERR_ARGUMENT, "range must be positive");
     HIPCHECK(c, hipSetDevice(c->device));
+    dropGraph(S);  // buffers may move
     S->pot = pot ? Potential{pot->kind, pot->p0, pot->p1, pot->p2} : Potential{POT_NONE, 0, 0, 0};
     S->range = range;
     S->cellList = cellList ? 1 : 0;
@@ -714,18 +797,8 @@ csg_status csg_system_nve(csg_ctx* c, double dt, int32_t nSteps) {
     if (!c) return CSG_ERR_ARGUMENT;
     if (!S || !S->configured) return fail(c, CSG_ERR_STATE, "csg_system_init and csg_system_configure first");
     HIPCHECK(c, hipSetDevice(c->device));
-    const int N = S->N;
-    for (int step = 0; step < nSteps; ++step) {
-        hipLaunchKernelGGL(kVVFirst, dim3(blocksFor(N)), dim3(kBlock), 0, c->stream, N, dt, S->vel.as<double>(), S->force.as<double>(),
-                           S->disp.as<double>());
-        csg_status s = sysMove(c, S);
-        if (s) return s;
-        s = sysForces(c, S);
-        if (s) return s;
-        hipLaunchKernelGGL(kVVKick, dim3(blocksFor(N)), dim3(kBlock), 0, c->stream, N, dt, S->vel.as<double>(), S->force.as<double>());
-        HIPCHECK(c, hipGetLastError());
-    }
-    return CSG_OK;
+    const double key[6] = {1, dt, 0, 0, 0, (double)S->ldsVariant};
+    return runSteps(c, S, key, nSteps, [=]() { return stepNve(c, S, dt); });
 }
 
 csg_status csg_system_nvt(csg_ctx* c, double dt, double T, double tau, int32_t M, int32_t nSteps) {
@@ -749,26 +822,8 @@ csg_status csg_system_nvt(csg_ctx* c, double dt, double T, double tau, int32_t M
         S->tau = tau;
         S->chainReady = true;
     }
-    const double dt2 = 0.5 * dt;
-    const int nb = blocksFor(N);
-    for (int step = 0; step < nSteps; ++step) {
-        hipLaunchKernelGGL(kNHChain, dim3(1), dim3(1), 0, c->stream, S->chain.as<double>(), S->ke.as<double>(), M, T, dt);
-        hipLaunchKernelGGL(kScaleVel, dim3(blocksFor(3 * N)), dim3(kBlock), 0, c->stream, N, S->vel.as<double>(), S->ke.as<double>());
-        hipLaunchKernelGGL(kScaledCopy, dim3(blocksFor(3 * N)), dim3(kBlock), 0, c->stream, N, dt2, S->vel.as<double>(), S->disp.as<double>());
-        csg_status s = sysMove(c, S);
-        if (s) return s;
-        s = sysForces(c, S);
-        if (s) return s;
-        hipLaunchKernelGGL(kNHKick, dim3(nb), dim3(kBlock), 0, c->stream, N, dt, dt2, S->vel.as<double>(), S->force.as<double>(),
-                           S->mass.as<double>(), S->disp.as<double>(), S->partial.as<double>());
-        hipLaunchKernelGGL(kSumPartials, dim3(1), dim3(1), 0, c->stream, nb, S->partial.as<double>(), S->ke.as<double>());
-        s = sysMove(c, S);
-        if (s) return s;
-        hipLaunchKernelGGL(kNHChain, dim3(1), dim3(1), 0, c->stream, S->chain.as<double>(), S->ke.as<double>(), M, T, dt);
-        hipLaunchKernelGGL(kScaleVel, dim3(blocksFor(3 * N)), dim3(kBlock), 0, c->stream, N, S->vel.as<double>(), S->ke.as<double>());
-        HIPCHECK(c, hipGetLastError());
-    }
-    return CSG_OK;
+    const double key[6] = {2, dt, T, tau, (double)M, (double)S->ldsVariant};
+    return runSteps(c, S, key, nSteps, [=]() { return stepNvt(c, S, dt, T, M); });
 }
 
 csg_status csg_system_sync(csg_ctx* c) {
@@ -907,11 +962,21 @@ csg_status csg_system_autotune(csg_ctx* c, int32_t* chosen, double* ms) {
     return CSG_OK;
 }
 
+csg_status csg_system_use_graph(csg_ctx* c, int32_t enable) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
+    dropGraph(S);
+    S->useGraph = enable != 0;
+    return CSG_OK;
+}
+
 csg_status csg_system_set_lds_variant(csg_ctx* c, int32_t variant) {
     SystemState* S = sysOf(c);
     if (!c) return CSG_ERR_ARGUMENT;
     if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
     if (variant < 0 || variant >= kLdsVariants) return fail(c, CSG_ERR_ARGUMENT, "unknown LDS kernel variant");
+    dropGraph(S);
     S->ldsVariant = variant;
     return CSG_OK;
 }
@@ -920,6 +985,7 @@ csg_status csg_system_record_pairs(csg_ctx* c, int32_t enable) {
     SystemState* S = sysOf(c);
     if (!c) return CSG_ERR_ARGUMENT;
     if (!S || !S->configured) return fail(c, CSG_ERR_STATE, "csg_system_init and csg_system_configure first");
+    dropGraph(S);
     HIPCHECK(c, hipSetDevice(c->device));
     if (enable) HIPCHECK(c, S->pdist.ensure(8 * (size_t)std::max<long>(S->candCap, 1)));
     S->recordPairs = enable != 0;
@@ -960,6 +1026,7 @@ csg_status csg_system_set_screen(csg_ctx* c, const float* params, float threshol
     if (!c) return CSG_ERR_ARGUMENT;
     if (!S || !S->configured) return fail(c, CSG_ERR_STATE, "csg_system_init and csg_system_configure first");
     if (params && (!S->cellList || !c->submeshing)) return fail(c, CSG_ERR_STATE, "the screen needs cell-list candidates and submeshing");
+    dropGraph(S);
     HIPCHECK(c, hipSetDevice(c->device));
     if (!params) {
         S->screenOn = false;

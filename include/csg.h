@@ -148,6 +148,10 @@ csg_status csg_system_fallback_counts(csg_ctx* ctx, int32_t* reasons);
  * chosen and ms[2] (per force computation) nullable. The forces of the state are recomputed, the particles are not
  * moved. */
 csg_status csg_system_autotune(csg_ctx* ctx, int32_t* chosen, double* ms);
+/* graph mode (no reference counterpart): csg_system_nve / csg_system_nvt capture one step as a HIP graph on first use
+ * for their parameters and replay it per step, which removes the per-launch host cost; ignored while event timing
+ * (csg_system_profile) is on. Any configuration change drops the captured step. */
+csg_status csg_system_use_graph(csg_ctx* ctx, int32_t enable);
 /* sets the LDS kernel variant directly (0 or 1, as above) */
 csg_status csg_system_set_lds_variant(csg_ctx* ctx, int32_t variant);
 /* candidate lists of the last force computation (offsets N+1, indices up to cap), before any screening */

@@ -5,8 +5,9 @@ Tolerances: logits 1e-4 + 1e-3 |z| against the restatement that rounds the same 
 FP32 accumulation order); gradients 1e-2 of each tensor's largest entry against that restatement (a bfloat16
 rounding of an FP32 intermediate may fall on the other side of a tie) and 0.2 in relative 2-norm against exact FP64
 (bfloat16 rounding, amplified where a gradient entry is a sum of few terms that cancel: up to 0.105 over 80 random
-draws of the restatement alone); loss 1e-5 relative. Screened forces equal plain forces to 1e-12 of the largest force
-(the sum order of the kept pairs may differ), and no dropped pair lies within the cutoff.
+draws of the restatement alone); loss 1e-5 relative. Screened forces equal plain forces to 1e-6 of the largest force
+(the sum order of the kept pairs may differ, and two windows that reach a target at the same FP64 distance may leave
+either's FP32 start tangent), energies to 1e-12, and no dropped pair lies within the cutoff.
 """
 import numpy as np
 import pytest
@@ -88,7 +89,7 @@ def test_screened_forces_match_plain(ctx):
     st = sysm.screen_stats()
     assert st["candidates"] == len(y) and st["kept"] == int(np.sum(~drop))
     scale = np.abs(F0).max()
-    assert np.max(np.abs(F1 - F0)) <= 1e-12 * scale
+    assert np.max(np.abs(F1 - F0)) <= 1e-6 * scale
     assert abs(E1 - E0) <= 1e-12 * abs(E0)
     print(f"screen: {built['train_pairs']} training pairs, threshold {built['threshold']:.3f}, "
           f"{np.sum(drop)} of {len(y)} pairs dropped ({np.mean(drop):.4%}); beyond cutoff {np.mean(y < 0.5):.4%}",

@@ -108,3 +108,33 @@ def test_lds_variants_and_autotune(ctx):
     chosen, ms = sysm.autotune()
     assert chosen in (0, 1) and np.all(ms > 0)
     assert sysm.status()["unsolved"] == 0
+
+
+@pytest.mark.parametrize("integrator", ["nve", "nvt"])
+def test_graph_replays_match_eager(integrator):
+    """Graph mode: one captured step replayed several times, with host reads and synchronisation between the replays,
+    against the same steps launched directly on an identical system (separate contexts). Tolerances: positions 1e-10,
+    velocities 1e-9 of their scale (the two runs may order equal-length window work differently)."""
+    print(f"case graph replays {integrator}", flush=True)
+    w = workloads.config(2, n=2000)
+    ca, cb = csa.Context(0), csa.Context(0)
+    try:
+        a = workloads.make_system(ca, w)
+        b = workloads.make_system(cb, w)
+        b.use_graph(True)
+        om = OracleMesh(w.X, w.T)
+        for rep in range(3):
+            if integrator == "nve":
+                a.nve(w.dt, 2)
+                b.nve(w.dt, 2)
+            else:
+                a.nvt(w.dt, 0.5, 1.0, 2, 2)
+                b.nvt(w.dt, 0.5, 1.0, 2, 2)
+            fa, ba, va, Fa = a.get()
+            fb, bb, vb, Fb = b.get()
+            np.testing.assert_allclose(om.positions(fb, bb), om.positions(fa, ba), atol=1e-10)
+            np.testing.assert_allclose(vb, va, atol=1e-9 * np.abs(va).max())
+        assert b.status()["unsolved"] == 0 and b.status()["walk_failures"] == 0
+    finally:
+        ca.close()
+        cb.close()
