@@ -172,7 +172,7 @@ def main():
     ctx = csa.Context(0)
     sysm = workloads.make_system(ctx, w)
     chosen, tune_ms = sysm.autotune()
-    log(f"autotune: LDS kernel variant {chosen} ({[256, 512][chosen]} lanes per source); ms per force computation {tune_ms}")
+    log(f"autotune: LDS kernel variant {chosen}; ms per force computation {tune_ms}")
     sysm.use_graph(not args.no_graph)
     log(f"warmup {args.warmup} steps")
     workloads.step(sysm, w, args.warmup)
@@ -222,8 +222,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "N": int(len(w.face)), "faces": int(len(w.T)), "cutoff": w.range,
                    "integrator": w.integrator, "dt": w.dt, "screen": "plain (no learned screen)",
-                   "lanes_per_source": [256, 512][chosen]},
-        "autotune_ms_per_force": {"256_lanes": float(tune_ms[0]), "512_lanes": float(tune_ms[1])},
+                   "lds_kernel_variant": ["256 lanes, 2 per CU", "512 lanes, 2 per CU", "256 lanes, 3 per CU"][chosen]},
+        "autotune_ms_per_force": {"256_lanes_2_per_cu": float(tune_ms[0]), "512_lanes_2_per_cu": float(tune_ms[1]),
+                                  "256_lanes_3_per_cu": float(tune_ms[2])},
         "graph_mode": not args.no_graph,
         "steps_per_s_direct_launches": args.steps / elapsed_eager,
         "pairs_per_step": int(pairs) * forces_per_step,

@@ -360,7 +360,7 @@ class System:
     def autotune(self):
         """Times each workgroup size of the LDS geodesic kernel and keeps the fastest: (chosen, ms per variant)."""
         v = C.c_int32()
-        ms = np.zeros(2)
+        ms = np.zeros(3)
         self.ctx._check(self.L.csg_system_autotune(self.ctx.h, C.byref(v), ms), "csg_system_autotune")
         return v.value, ms
 

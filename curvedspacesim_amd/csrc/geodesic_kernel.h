@@ -79,8 +79,10 @@ struct ScratchLayout {
 ScratchLayout makeScratchLayout(long faces, long targets);
 // face sets of every source (requires P.patchRadius); stride <= kPatchListCap
 hipError_t launchPatch(const GeoParams& P, int32_t* list, int32_t* count, int stride, hipStream_t stream);
-// LDS variants: 0 = 256 lanes (4 wavefronts) per source, 1 = 512 lanes (8 wavefronts); both run two workgroups per CU
-constexpr int kLdsVariants = 2;
+// LDS variants: 0 = 256 lanes (4 wavefronts) per source and 1 = 512 lanes (8 wavefronts), both two workgroups per CU
+// with the capacities above; 2 = 256 lanes, three workgroups per CU with 576 faces, 352 vertices, a 384-record ring
+// and 224 targets
+constexpr int kLdsVariants = 3;
 hipError_t launchGeodesicLds(const GeoParams& P, int variant, hipStream_t stream);
 hipError_t launchGeodesicScratch(const GeoParams& P, const ScratchLayout& L, const int32_t* list, int nList, int blocks,
                                  hipStream_t stream);

@@ -1008,11 +1008,11 @@ __device__ inline int sourceOfBlock(const GeoParams& P) {
 
 int gridForSources(const GeoParams& P) { return P.blockOrder ? kXcds * P.blockChunk : P.nSrc; }
 
-// LDS variant: tables in __shared__ memory, one source per workgroup, two workgroups per CU (at most 80 KB each).
-// FC faces, VC vertices, QC ring records, TC targets; HF and HV face and vertex hash slots, which share the ring's
-// storage; kEnd: the variant that also reports end tangents.
-template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, bool kEnd>
-__global__ __launch_bounds__(NT, 2) void geodesicForceLds(GeoParams P) {
+// LDS variant: tables in __shared__ memory, one source per workgroup of NT lanes, WGS workgroups per CU (each within
+// 160 KB / WGS of LDS). FC faces, VC vertices, QC ring records, TC targets; HF and HV face and vertex hash slots, which
+// share the ring's storage; kEnd: the variant that also reports end tangents.
+template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, int WGS, bool kEnd>
+__global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P) {
     constexpr int kRingBytes = QC * (5 * 8 + 4 + 8);
     static_assert(HF * 6 + HV * 6 <= kRingBytes, "setup hashes live in the ring's storage");
     static_assert(QC % 2 == 0 && (HF & (HF - 1)) == 0 && (HV & (HV - 1)) == 0, "alignment and power-of-two hashes");
@@ -1199,22 +1199,22 @@ ScratchLayout makeScratchLayout(long faces, long targets) {
     return L;
 }
 
-template <int NT>
+template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, int WGS>
 static void launchLdsVariant(const GeoParams& P, hipStream_t stream) {
     if (P.pairEnd)
-        hipLaunchKernelGGL((geodesicForceLds<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsFaceHash, kLdsVertHash, NT, true>),
-                           dim3(gridForSources(P)), dim3(NT), 0, stream, P);
+        hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, true>), dim3(gridForSources(P)), dim3(NT), 0, stream, P);
     else
-        hipLaunchKernelGGL((geodesicForceLds<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsFaceHash, kLdsVertHash, NT, false>),
-                           dim3(gridForSources(P)), dim3(NT), 0, stream, P);
+        hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, false>), dim3(gridForSources(P)), dim3(NT), 0, stream, P);
 }
 
 hipError_t launchGeodesicLds(const GeoParams& P, int variant, hipStream_t stream) {
     if (P.nSrc <= 0) return hipSuccess;
     if (variant == 1)
-        launchLdsVariant<512>(P, stream);
+        launchLdsVariant<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsFaceHash, kLdsVertHash, 512, 2>(P, stream);
+    else if (variant == 2)  // smaller tables and ring, three workgroups per CU
+        launchLdsVariant<576, 352, 384, 224, 1024, 512, 256, 3>(P, stream);
     else
-        launchLdsVariant<256>(P, stream);
+        launchLdsVariant<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsFaceHash, kLdsVertHash, 256, 2>(P, stream);
     return hipGetLastError();
 }
 

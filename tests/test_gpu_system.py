@@ -93,20 +93,21 @@ def test_nvt_full_size_runs(ctx):
 
 
 def test_lds_variants_and_autotune(ctx):
-    """Both workgroup sizes of the LDS kernel give the same forces (to the sum order of a source's pairs), and the
-    autotuner keeps one of them."""
+    """Every variant of the LDS kernel gives the same forces (to the sum order of a source's pairs and FP32 tangents of
+    tied windows), and the autotuner keeps one of them."""
     w = workloads.config(3, n=4000)
     print("case lds variants config 3 N=4000", flush=True)
     sysm = workloads.make_system(ctx, w)
     out = []
-    for v in (0, 1):
+    for v in (0, 1, 2):
         sysm.set_lds_variant(v)
         out.append(sysm.compute_forces())
     scale = np.abs(out[0][0]).max()
-    assert np.max(np.abs(out[1][0] - out[0][0])) <= 1e-12 * scale
-    assert abs(out[1][1] - out[0][1]) <= 1e-12 * abs(out[0][1])
+    for F, E in out[1:]:
+        assert np.max(np.abs(F - out[0][0])) <= 1e-6 * scale
+        assert abs(E - out[0][1]) <= 1e-12 * abs(out[0][1])
     chosen, ms = sysm.autotune()
-    assert chosen in (0, 1) and np.all(ms > 0)
+    assert chosen in (0, 1, 2) and np.all(ms > 0)
     assert sysm.status()["unsolved"] == 0
 
 
