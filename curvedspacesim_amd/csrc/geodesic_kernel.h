@@ -18,9 +18,10 @@ constexpr int kLdsVertHash = 1024;
 // work items beyond the ring wait in a per-source HBM stack of this many records
 constexpr int kSpillCap = 1024;
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X, each with its own L2
-// patch kernel: hash slots per source and capacity of the written face list
-constexpr int kPatchHashCap = 2048;
-constexpr int kPatchListCap = 1024;
+// patch kernel: hash slots per source, capacity of the written face list, and of one breadth-first level
+constexpr int kPatchHashCap = 1024;
+constexpr int kPatchListCap = 768;
+constexpr int kPatchFrontCap = 512;
 
 // reasons a source exceeded the capacities of a kernel variant (bits of the per-source status)
 enum OverflowReason : int { OVF_FACES = 1, OVF_VERTS = 2, OVF_QUEUE = 4, OVF_TARGETS = 8, OVF_STEPS = 16, OVF_FRONTIER = 32 };

@@ -172,6 +172,20 @@ __device__ void faceInsert(const TT& T, Shared& sh, int g, bool& fresh) {
     atomicOr(&sh.overflow, OVF_FACES);
 }
 
+// inserts a face known to be absent with a given slot (faces of a prepared list)
+template <class TT>
+__device__ void faceInsertAt(const TT& T, int g, int slot) {
+    uint32_t h = hashOf((uint32_t)g) & T.hmask;
+    for (int p = 0; p <= T.hmask; ++p) {
+        if (atomicCAS(&T.hkey[h], -1, g) == -1) {
+            T.hslot[h] = slot;
+            T.fglob[slot] = g;
+            return;
+        }
+        h = (h + 1) & T.hmask;
+    }
+}
+
 template <class TT>
 __device__ void vertInsert(const TT& T, Shared& sh, int v) {
     uint32_t h = hashOf((uint32_t)v) & T.vhmask;
@@ -612,11 +626,13 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int* cur
             bool fresh;
             faceInsert(T, sh, g, fresh);
         }
-    } else if (listed >= 0) {  // prepared by the patch kernel
+    } else if (listed >= 0) {  // prepared by the patch kernel (distinct faces): slot i is the list's i-th face
         const int32_t* lst = P.patchList + (size_t)src * P.patchStride;
-        for (int i = lane; i < listed; i += blockDim.x) {
-            bool fresh;
-            faceInsert(T, sh, lst[i], fresh);
+        if (listed > T.fcap) {
+            if (lane == 0) atomicOr(&sh.overflow, OVF_FACES);
+        } else {
+            for (int i = lane; i < listed; i += blockDim.x) faceInsertAt(T, lst[i], i);
+            if (lane == 0) sh.nFaces = listed;
         }
     } else if (cur) {
         patchSearch(P, T, sh, cur, nxt, frontCap, src, sf, tb, nT);
@@ -1127,9 +1143,9 @@ __global__ __launch_bounds__(64) void geodesicForceScratch(GeoParams P, ScratchL
     }
 }
 
-// Patch kernel: the reference's submesh face set of every source, one wavefront and ~24 KB of LDS per source (so
-// several sources per CU hide the search's memory latency), written as a compact list (count -1: more faces than the
-// list holds).
+// Patch kernel: the reference's submesh face set of every source, one wavefront and ~13 KB of LDS per source (so
+// a dozen sources per CU hide the search's memory latency), written as a compact list (count -1: more faces than the
+// list holds, or a breadth-first level wider than its buffer).
 struct FaceSet {
     int32_t* hkey;
     int16_t* hslot;
@@ -1141,7 +1157,7 @@ __global__ __launch_bounds__(64) void patchKernel(GeoParams P, int32_t* list, in
     __shared__ int32_t hkey[kPatchHashCap];
     __shared__ int16_t hslot[kPatchHashCap];
     __shared__ int32_t fglob[kPatchListCap];
-    __shared__ int32_t cur[kPatchListCap], nxt[kPatchListCap];
+    __shared__ int32_t cur[kPatchFrontCap], nxt[kPatchFrontCap];
     __shared__ Shared sh;
     const int src = sourceOfBlock(P);
     if (src < 0) return;
@@ -1156,7 +1172,7 @@ __global__ __launch_bounds__(64) void patchKernel(GeoParams P, int32_t* list, in
         initSourceFrame(P, sh, pi, sf);
     }
     __syncthreads();
-    patchSearch(P, T, sh, cur, nxt, kPatchListCap, src, sf, tb, nT);
+    patchSearch(P, T, sh, cur, nxt, kPatchFrontCap, src, sf, tb, nT);
     __syncthreads();
     if (sh.overflow) {
         if (lane == 0) count[src] = -1;
