@@ -179,54 +179,79 @@ __global__ void kScatter(int N, const int* key, const int* cellStart, int* cellF
 
 // particles of a cell in increasing index (the reference fills cells in particle order)
 __global__ void kSortCells(int nCells, const int* cellStart, int* order) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    // one wavefront per cell: every element's rank is the number of smaller ones (indices are distinct), found by
+    // broadcasting the cell's elements across the lanes; all reads finish before the writes (one wavefront per cell)
+    const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
     if (c >= nCells) return;
-    int b = cellStart[c], e = cellStart[c + 1];
-    for (int i = b + 1; i < e; ++i) {
-        int v = order[i], j = i - 1;
-        while (j >= b && order[j] > v) {
-            order[j + 1] = order[j];
-            --j;
-        }
-        order[j + 1] = v;
+    const int b = cellStart[c], n = cellStart[c + 1] - b;
+    if (n <= 1) return;
+    if (n <= 64) {
+        const int v = lane < n ? order[b + lane] : 0x7fffffff;
+        int rank = 0;
+        for (int j = 0; j < n; ++j) rank += __shfl(v, j, 64) < v ? 1 : 0;
+        if (lane < n) order[b + rank] = v;
+        return;
     }
+    if (lane == 0)  // a dense cell: insertion sort by one lane
+        for (int i = b + 1; i < b + n; ++i) {
+            int v = order[i], j = i - 1;
+            while (j >= b && order[j] > v) {
+                order[j + 1] = order[j];
+                --j;
+            }
+            order[j + 1] = v;
+        }
 }
 
-template <bool kFill>
 // Candidates of every particle in the reference's order (stencil cells x, then y, then z; ascending index within a
-// cell). Thread t takes the t-th particle in cell order, so a wavefront's particles share stencil cells, and the
-// positions are read from their cell-ordered copy sxyz, contiguous within a cell.
-__global__ void kCandidates(Grid g, int N, const double* sxyz, const int* key, const int* cellStart, const int* order, double r2, int* cnt,
-                            const int* off, long cap, int* cand, double* maxd, double* rad, double maxDist, int* counters) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+// cell), one wavefront per particle: the lanes test consecutive members of each stencil cell, and a ballot prefix places
+// the hits in that order. Wavefront t takes the t-th particle in cell order and reads positions from their cell-ordered
+// copy sxyz, contiguous within a cell.
+template <bool kFill>
+__global__ void kCandidatesWave(Grid g, int N, const double* sxyz, const int* key, const int* cellStart, const int* order, double r2,
+                                int* cnt, const int* off, long cap, int* cand, double* maxd, double* rad, double maxDist, int* counters) {
+    const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
     if (t >= N) return;
     const int i = order[t];
     const double p[3] = {sxyz[3 * t], sxyz[3 * t + 1], sxyz[3 * t + 2]};
-    int c = key[i];
-    int cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / (g.n[0] * g.n[1]);
-    int k = 0;
+    const int c = key[i];
+    const int cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / (g.n[0] * g.n[1]);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    long k = 0;
     double m2 = 0;
-    long base = kFill ? off[i] : 0;
+    const long base = kFill ? off[i] : 0;
     for (int x = max(0, cx - 1); x <= min(g.n[0] - 1, cx + 1); ++x)
         for (int y = max(0, cy - 1); y <= min(g.n[1] - 1, cy + 1); ++y)
             for (int z = max(0, cz - 1); z <= min(g.n[2] - 1, cz + 1); ++z) {
-                int cc = x + g.n[0] * (y + g.n[1] * z);
-                for (int q = cellStart[cc]; q < cellStart[cc + 1]; ++q) {
-                    if (q == t) continue;
-                    double dx = p[0] - sxyz[3 * q], dy = p[1] - sxyz[3 * q + 1], dz = p[2] - sxyz[3 * q + 2];
-                    double d2 = dx * dx + dy * dy + dz * dz;
-                    if (d2 < r2) {
-                        if (kFill && base + k < cap) cand[base + k] = order[q];
-                        ++k;
+                const int cc = x + g.n[0] * (y + g.n[1] * z);
+                const int ce = cellStart[cc + 1];
+                for (int q0 = cellStart[cc]; q0 < ce; q0 += 64) {
+                    const int q = q0 + lane;
+                    bool hit = false;
+                    double d2 = 0;
+                    if (q < ce && q != t) {
+                        const double dx = p[0] - sxyz[3 * q], dy = p[1] - sxyz[3 * q + 1], dz = p[2] - sxyz[3 * q + 2];
+                        d2 = dx * dx + dy * dy + dz * dz;
+                        hit = d2 < r2;
+                    }
+                    const unsigned long long mask = __ballot(hit);
+                    if (hit) {
+                        const long pos = k + __popcll(mask & below);
+                        if (kFill && base + pos < cap) cand[base + pos] = order[q];
                         if (d2 > m2) m2 = d2;
                     }
+                    k += __popcll(mask);
                 }
             }
+    for (int o = 32; o > 0; o >>= 1) m2 = fmax(m2, __shfl_xor(m2, o, 64));
+    if (lane != 0) return;
     if (!kFill) {
-        cnt[i] = k;
+        cnt[i] = (int)k;
     } else {
         if (base + k > cap) atomicAdd(&counters[CNT_CAND_OVERFLOW], 1);
-        double md = sqrt(m2);
+        const double md = sqrt(m2);
         maxd[i] = md;
         rad[i] = md < maxDist ? md : maxDist;
     }
@@ -432,15 +457,15 @@ static csg_status sysNeighbors(csg_ctx* c, SystemState* S) {
     hipLaunchKernelGGL(kScanExclusive, dim3(1), dim3(1024), 0, st, S->cellCount.as<int>(), S->cellStart.as<int>(), S->nCells);
     hipLaunchKernelGGL(kScatter, dim3(blocksFor(N)), dim3(kBlock), 0, st, N, S->key.as<int>(), S->cellStart.as<int>(), S->cellFill.as<int>(),
                        S->order.as<int>());
-    hipLaunchKernelGGL(kSortCells, dim3(blocksFor(S->nCells)), dim3(kBlock), 0, st, S->nCells, S->cellStart.as<int>(), S->order.as<int>());
+    hipLaunchKernelGGL(kSortCells, dim3(blocksFor(64L * S->nCells)), dim3(kBlock), 0, st, S->nCells, S->cellStart.as<int>(), S->order.as<int>());
     hipLaunchKernelGGL(kGatherPositions, dim3(blocksFor(N)), dim3(kBlock), 0, st, N, S->order.as<int>(), S->xyz.as<double>(),
                        S->sxyz.as<double>());
     double r2 = S->range * S->range;
-    hipLaunchKernelGGL((kCandidates<false>), dim3(blocksFor(N)), dim3(kBlock), 0, st, g, N, S->sxyz.as<double>(), S->key.as<int>(),
+    hipLaunchKernelGGL((kCandidatesWave<false>), dim3(blocksFor(64L * N)), dim3(kBlock), 0, st, g, N, S->sxyz.as<double>(), S->key.as<int>(),
                        S->cellStart.as<int>(), S->order.as<int>(), r2, S->cnt.as<int>(), (const int*)nullptr, S->candCap, (int*)nullptr,
                        (double*)nullptr, (double*)nullptr, c->maximumDistance, S->counters.as<int>());
     hipLaunchKernelGGL(kScanExclusive, dim3(1), dim3(1024), 0, st, S->cnt.as<int>(), S->off.as<int>(), N);
-    hipLaunchKernelGGL((kCandidates<true>), dim3(blocksFor(N)), dim3(kBlock), 0, st, g, N, S->sxyz.as<double>(), S->key.as<int>(),
+    hipLaunchKernelGGL((kCandidatesWave<true>), dim3(blocksFor(64L * N)), dim3(kBlock), 0, st, g, N, S->sxyz.as<double>(), S->key.as<int>(),
                        S->cellStart.as<int>(), S->order.as<int>(), r2, S->cnt.as<int>(), S->off.as<int>(), S->candCap, S->cand.as<int>(),
                        S->maxd.as<double>(), S->rad.as<double>(), c->maximumDistance, S->counters.as<int>());
     hipLaunchKernelGGL(kClampOffsets, dim3(blocksFor(N + 1)), dim3(kBlock), 0, st, N, S->off.as<int>(), S->candCap, S->toff.as<int>(),
