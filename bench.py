@@ -222,9 +222,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "N": int(len(w.face)), "faces": int(len(w.T)), "cutoff": w.range,
                    "integrator": w.integrator, "dt": w.dt, "screen": "plain (no learned screen)",
-                   "lds_kernel_variant": ["256 lanes, 2 per CU", "512 lanes, 2 per CU", "256 lanes, 3 per CU"][chosen]},
-        "autotune_ms_per_force": {"256_lanes_2_per_cu": float(tune_ms[0]), "512_lanes_2_per_cu": float(tune_ms[1]),
-                                  "256_lanes_3_per_cu": float(tune_ms[2])},
+                   "lds_kernel_variant": csa.LDS_VARIANT_NAMES[chosen]},
+        "autotune_ms_per_force": {n: float(t) for n, t in zip(csa.LDS_VARIANT_NAMES, tune_ms)},
         "graph_mode": not args.no_graph,
         "steps_per_s_direct_launches": args.steps / elapsed_eager,
         "pairs_per_step": int(pairs) * forces_per_step,

@@ -30,6 +30,9 @@ class csg_potential(C.Structure):
 HARMONIC, GAUSSIAN, TRUNC_GAUSSIAN, NO_POTENTIAL = 0, 1, 2, 3
 PAIR_UNREACHED, PAIR_BEYOND, PAIR_SENTINEL = 1, 4, 16
 SCREEN_FEATURES, SCREEN_PARAMS = 16, 5313
+# variants of the LDS geodesic kernel (CSG_LDS_VARIANTS), in the library's order
+LDS_VARIANT_NAMES = ["256_lanes_2_per_cu", "512_lanes_2_per_cu", "256_lanes_3_per_cu", "256_lanes_4_per_cu"]
+LDS_VARIANTS = len(LDS_VARIANT_NAMES)
 
 
 def _path(name):
@@ -360,7 +363,7 @@ class System:
     def autotune(self):
         """Times each workgroup size of the LDS geodesic kernel and keeps the fastest: (chosen, ms per variant)."""
         v = C.c_int32()
-        ms = np.zeros(3)
+        ms = np.zeros(LDS_VARIANTS)
         self.ctx._check(self.L.csg_system_autotune(self.ctx.h, C.byref(v), ms), "csg_system_autotune")
         return v.value, ms
 

@@ -1,6 +1,7 @@
 // Launch interface of the batched geodesic + pair-force kernels (geodesic_kernel.hip).
 #pragma once
 
+#include "../../include/csg.h"
 #include "csg_common.h"
 
 namespace csg {
@@ -82,8 +83,9 @@ ScratchLayout makeScratchLayout(long faces, long targets);
 hipError_t launchPatch(const GeoParams& P, int32_t* list, int32_t* count, int stride, hipStream_t stream);
 // LDS variants: 0 = 256 lanes (4 wavefronts) per source and 1 = 512 lanes (8 wavefronts), both two workgroups per CU
 // with the capacities above; 2 = 256 lanes, three workgroups per CU with 576 faces, 352 vertices, a 384-record ring
-// and 224 targets
-constexpr int kLdsVariants = 3;
+// and 224 targets; 3 = 256 lanes, four workgroups per CU (40 KB each) with 576 faces, 320 vertices, a 240-record ring
+// and 128 targets
+constexpr int kLdsVariants = CSG_LDS_VARIANTS;
 hipError_t launchGeodesicLds(const GeoParams& P, int variant, hipStream_t stream);
 hipError_t launchGeodesicScratch(const GeoParams& P, const ScratchLayout& L, const int32_t* list, int nList, int blocks,
                                  hipStream_t stream);

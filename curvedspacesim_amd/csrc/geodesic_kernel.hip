@@ -1229,6 +1229,8 @@ hipError_t launchGeodesicLds(const GeoParams& P, int variant, hipStream_t stream
         launchLdsVariant<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsFaceHash, kLdsVertHash, 512, 2>(P, stream);
     else if (variant == 2)  // smaller tables and ring, three workgroups per CU
         launchLdsVariant<576, 352, 384, 224, 1024, 512, 256, 3>(P, stream);
+    else if (variant == 3)  // four workgroups per CU: 40 KB each
+        launchLdsVariant<576, 320, 240, 128, 1024, 512, 256, 4>(P, stream);
     else
         launchLdsVariant<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsFaceHash, kLdsVertHash, 256, 2>(P, stream);
     return hipGetLastError();

@@ -143,17 +143,20 @@ csg_status csg_system_patch_stats(csg_ctx* ctx, int64_t* sums, int32_t* maxima);
 /* sources handed from the LDS kernel to the HBM kernel since csg_system_init, by reason: reasons[6] = {patch faces,
  * patch vertices, live windows, targets, step limit, search frontier} */
 csg_status csg_system_fallback_counts(csg_ctx* ctx, int32_t* reasons);
-/* autotuner (no reference counterpart): times the force computation of the current state with each variant of the LDS
- * geodesic kernel (0: 256 lanes per source, 1: 512 lanes, both two workgroups per CU; 2: 256 lanes, three workgroups
- * per CU with smaller tables and work-item ring) and keeps the fastest for later steps; chosen and ms[3] (per force
- * computation) nullable. The forces of the state are recomputed, the particles are not
- * moved. */
+/* variants of the LDS geodesic kernel: 0 and 1: 256 and 512 lanes per source, two workgroups per CU; 2: 256 lanes, three
+ * workgroups per CU with smaller tables and work-item ring; 3: 256 lanes, four workgroups per CU (40 KB of LDS each:
+ * 320 patch vertices, 128 targets, a 240-record ring). Sources that do not fit a variant's tables are solved by the
+ * HBM-table kernel. */
+#define CSG_LDS_VARIANTS 4
+/* autotuner (no reference counterpart): times the force computation of the current state with each LDS kernel variant
+ * and keeps the fastest for later steps; chosen and ms[CSG_LDS_VARIANTS] (per force computation) nullable. The forces
+ * of the state are recomputed, the particles are not moved. */
 csg_status csg_system_autotune(csg_ctx* ctx, int32_t* chosen, double* ms);
 /* graph mode (no reference counterpart): csg_system_nve / csg_system_nvt capture one step as a HIP graph on first use
  * for their parameters and replay it per step, which removes the per-launch host cost; ignored while event timing
  * (csg_system_profile) is on. Any configuration change drops the captured step. */
 csg_status csg_system_use_graph(csg_ctx* ctx, int32_t enable);
-/* sets the LDS kernel variant directly (0, 1 or 2, as above) */
+/* sets the LDS kernel variant directly (0 .. CSG_LDS_VARIANTS - 1, as above) */
 csg_status csg_system_set_lds_variant(csg_ctx* ctx, int32_t variant);
 /* candidate lists of the last force computation (offsets N+1, indices up to cap), before any screening */
 csg_status csg_system_pairs(csg_ctx* ctx, int32_t* offsets, int32_t* idx, int64_t cap);

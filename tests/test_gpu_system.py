@@ -99,7 +99,7 @@ def test_lds_variants_and_autotune(ctx):
     print("case lds variants config 3 N=4000", flush=True)
     sysm = workloads.make_system(ctx, w)
     out = []
-    for v in (0, 1, 2):
+    for v in range(csa.LDS_VARIANTS):
         sysm.set_lds_variant(v)
         out.append(sysm.compute_forces())
     scale = np.abs(out[0][0]).max()
@@ -107,7 +107,7 @@ def test_lds_variants_and_autotune(ctx):
         assert np.max(np.abs(F - out[0][0])) <= 1e-6 * scale
         assert abs(E - out[0][1]) <= 1e-12 * abs(out[0][1])
     chosen, ms = sysm.autotune()
-    assert chosen in (0, 1, 2) and np.all(ms > 0)
+    assert 0 <= chosen < csa.LDS_VARIANTS and len(ms) == csa.LDS_VARIANTS and np.all(ms > 0)
     assert sysm.status()["unsolved"] == 0
 
 
