@@ -163,11 +163,14 @@ csg_status preprocessMesh(csg_ctx* c, int nV, const double* X, int nF, const int
     HIPCHECK(c, upload(c, c->vflist, vflist.data(), vflist.size()));
     std::vector<double> curv = vertexCurvatures(nV, X, nF, F, angle, flag);
     HIPCHECK(c, upload(c, c->curv, curv.data(), curv.size()));
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, c->fchild.ensure(18 * sizeof(double) * (size_t)nF));
     c->nV = nV;
     c->nF = nF;
     c->dm = csg::DeviceMesh{nV, nF, c->vx.as<double>(), c->fv.as<int32_t>(), c->fadj.as<int32_t>(), c->fgeo.as<double>(),
-                            c->fnrm.as<double>(), c->vflag.as<uint8_t>(), c->vfoff.as<int32_t>(), c->vflist.as<int32_t>()};
+                            c->fnrm.as<double>(), c->vflag.as<uint8_t>(), c->vfoff.as<int32_t>(), c->vflist.as<int32_t>(),
+                            c->fchild.as<double>()};
+    HIPCHECK(c, csg::launchChildFrames(c->dm, c->fchild.as<double>(), c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
     c->haveMesh = true;
     return CSG_OK;
 }

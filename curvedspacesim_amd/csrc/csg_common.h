@@ -20,6 +20,10 @@ struct DeviceMesh {
     const uint8_t* __restrict__ vflag; // nV: bit0 saddle (angle sum >= 2 pi), bit1 mesh border
     const int32_t* __restrict__ vfoff; // nV + 1
     const int32_t* __restrict__ vflist;// face * 4 + local corner index
+    // 18 nF: per face and local edge k, the frames of the two edges a window entering through edge k leaves by:
+    // {lL, uxL, uyL, lR, uxR, uyR}, the length and unit direction (in the edge-k frame) of corner k -> corner k+2 and
+    // of corner k+2 -> corner k+1
+    const double* __restrict__ fchild;
 };
 
 // pair potential, reference conventions (src/forces/harmonicRepulsion.cpp, src/forces/gaussianRepulsion.cpp)

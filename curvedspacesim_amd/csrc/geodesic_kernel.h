@@ -87,6 +87,8 @@ hipError_t launchPatch(const GeoParams& P, int32_t* list, int32_t* count, int st
 // and 128 targets
 constexpr int kLdsVariants = CSG_LDS_VARIANTS;
 hipError_t launchGeodesicLds(const GeoParams& P, int variant, hipStream_t stream);
+// fills mesh.fchild (out, 18 per face) from mesh.fgeo
+hipError_t launchChildFrames(const DeviceMesh& mesh, double* out, hipStream_t stream);
 hipError_t launchGeodesicScratch(const GeoParams& P, const ScratchLayout& L, const int32_t* list, int nList, int blocks,
                                  hipStream_t stream);
 // scratch variant over every source whose P.status is non-zero (left unsolved by the LDS variant); no host round trip

@@ -263,22 +263,23 @@ __device__ inline void offerVertex(const TT& T, VCand& c, int slot, double d, fl
     c.t = t;
 }
 
-__device__ inline double keyOf(double b0, double b1, double sx, double sy, double sg) {
-    double dm;
-    if (sx < b0)
-        dm = sqrt((b0 - sx) * (b0 - sx) + sy * sy);
-    else if (sx > b1)
-        dm = sqrt((b1 - sx) * (b1 - sx) + sy * sy);
-    else
-        dm = -sy;
+// distance from the window's source (sx, sy) to the point b of its edge
+__device__ inline double endDist(double b, double sx, double sy) { return sqrt((b - sx) * (b - sx) + sy * sy); }
+
+// lower bound of the distances the window carries: its source distance plus the distance to the nearest interval point
+// (r0, r1: endDist of b0 and b1)
+__device__ inline double keyOf(double b0, double b1, double sx, double sy, double sg, double r0, double r1) {
+    const double dm = sx < b0 ? r0 : (sx > b1 ? r1 : -sy);
     return sg + dm;
 }
 
+// Xin-Wang filter against the edge's end vertices: the window is useless when one of its interval ends is reached
+// more cheaply through the vertex at the other end of the edge
 template <class TT>
-__device__ inline bool filtered(const TT& T, int vA, int vB, double L, double b0, double b1, double sx, double sy, double sg) {
+__device__ inline bool filtered(const TT& T, int vA, int vB, double L, double b0, double b1, double sg, double r0, double r1) {
     double dA = bitsd(T.vdist[vA]), dB = bitsd(T.vdist[vB]);
-    double toI1 = sg + sqrt((b1 - sx) * (b1 - sx) + sy * sy);
-    double toI0 = sg + sqrt((b0 - sx) * (b0 - sx) + sy * sy);
+    double toI1 = sg + r1;
+    double toI0 = sg + r0;
     if (toI1 > dA + b1 + kFilterTol * toI1) return true;
     if (toI0 > dB + (L - b0) + kFilterTol * toI0) return true;
     return false;
@@ -340,13 +341,15 @@ template <class TT>
 __device__ void addWindow(const Ctx<TT>& C, int fs, int k, double L, double b0, double b1, double sx, double sy, double sg, bool rooted,
                           float2 rot, VCand& cA, VCand& cB) {
     if (!(b1 > b0)) return;
-    if (keyOf(b0, b1, sx, sy, sg) > C.rMax) return;
+    const double r0 = endDist(b0, sx, sy), r1 = endDist(b1, sx, sy);
+    if (keyOf(b0, b1, sx, sy, sg, r0, r1) > C.rMax) return;
     const TT& T = C.T;
     int vA = T.fcv[3 * fs + k], vB = T.fcv[3 * fs + (k + 1) % 3];
     double tol = kEdgeTol * L;
-    if (b0 <= tol) offerVertex(T, cA, vA, sg + sqrt(sx * sx + sy * sy), startTangent(rooted, rot, -sx, -sy));
-    if (b1 >= L - tol) offerVertex(T, cB, vB, sg + sqrt((L - sx) * (L - sx) + sy * sy), startTangent(rooted, rot, L - sx, -sy));
-    if (filtered(T, vA, vB, L, b0, b1, sx, sy, sg)) return;
+    // an interval end on an edge end: the distance to that vertex (r0 or r1 when the end is exactly the vertex)
+    if (b0 <= tol) offerVertex(T, cA, vA, sg + (b0 == 0.0 ? r0 : endDist(0.0, sx, sy)), startTangent(rooted, rot, -sx, -sy));
+    if (b1 >= L - tol) offerVertex(T, cB, vB, sg + (b1 == L ? r1 : endDist(L, sx, sy)), startTangent(rooted, rot, L - sx, -sy));
+    if (filtered(T, vA, vB, L, b0, b1, sg, r0, r1)) return;
     pushItem(C, b0, b1, sx, sy, sg, fs * 8 + k * 2 + (rooted ? 1 : 0), rot);
 }
 
@@ -377,7 +380,7 @@ __device__ inline void pointInFrame(const DeviceMesh& M, int g, int k, const dou
 // One window against the targets of the face it enters. Pass 1: atomic minimum. Pass 2: tangents of the winner.
 template <bool kPass2, class TT>
 __device__ void windowTargets(const Ctx<TT>& C, int fs, int g, int k, double L, double x2, double y2, double b0, double b1, double sx,
-                              double sy, double sg, bool rooted, float2 rot) {
+                              double sy, double sg, double r0, double r1, bool rooted, float2 rot) {
     const TT& T = C.T;
     const GeoParams& P = C.P;
     for (int t = T.fhead[fs]; t >= 0; t = T.tnext[t]) {
@@ -390,7 +393,7 @@ __device__ void windowTargets(const Ctx<TT>& C, int fs, int g, int k, double L, 
         double c1 = bx * ry - by * rx;  // cross(interval end ray, target ray)
         double c2 = rx * ay - ry * ax;  // cross(target ray, interval start ray)
         double tol = 1e-12 * rl;
-        if (c1 < -tol * sqrt(bx * bx + by * by) || c2 < -tol * sqrt(ax * ax + ay * ay)) continue;
+        if (c1 < -tol * r1 || c2 < -tol * r0) continue;  // r1 = |(bx, by)|, r0 = |(ax, ay)|
         double d = sg + rl;
         if (!kPass2) {
             atomicMin(&T.tbest[t], dbits(d));
@@ -413,12 +416,12 @@ __device__ void windowTargets(const Ctx<TT>& C, int fs, int g, int k, double L, 
 // Window propagation across the face it enters (pass 1): candidate for the opposite corner, up to two children.
 template <class TT>
 __device__ void propagate(const Ctx<TT>& C, int fk, int g, double L, double x2, double y2, double b0, double b1, double sx, double sy,
-                          double sg, float2 rot, VCand& cP0, VCand& cP1, VCand& cP2) {
+                          double sg, double r0, double r1, const double* cf, float2 rot, VCand& cP0, VCand& cP1, VCand& cP2) {
     const TT& T = C.T;
     int fs = fk >> 3, k = (fk >> 1) & 3;
     bool rooted = fk & 1;
     int vC = T.fcv[3 * fs + (k + 2) % 3];
-    windowTargets<false, TT>(C, fs, g, k, L, x2, y2, b0, b1, sx, sy, sg, rooted, rot);
+    windowTargets<false, TT>(C, fs, g, k, L, x2, y2, b0, b1, sx, sy, sg, r0, r1, rooted, rot);
     double xs = sx + (x2 - sx) * (-sy) / (y2 - sy);  // where the ray through the opposite corner crosses the entry edge
     double tolE = kEdgeTol * L;
     if (xs >= b0 - tolE && xs <= b1 + tolE)
@@ -426,8 +429,7 @@ __device__ void propagate(const Ctx<TT>& C, int fk, int g, double L, double x2, 
     if (xs > b0) {  // left child: edge corner k -> corner k+2, entered from the neighbour across local edge k+2
         int nb = T.fnb[3 * fs + (k + 2) % 3];
         if (nb >= 0) {
-            double lL = sqrt(x2 * x2 + y2 * y2);
-            double ux = x2 / lL, uy = y2 / lL;
+            const double lL = cf[0], ux = cf[1], uy = cf[2];
             double spx = sx * ux + sy * uy, spy = ux * sy - uy * sx;
             auto hit = [&](double bx) {
                 double dx = bx - sx, dy = -sy;
@@ -444,9 +446,7 @@ __device__ void propagate(const Ctx<TT>& C, int fk, int g, double L, double x2, 
     if (xs < b1) {  // right child: edge corner k+2 -> corner k+1, entered from the neighbour across local edge k+1
         int nb = T.fnb[3 * fs + (k + 1) % 3];
         if (nb >= 0) {
-            double ex = L - x2, ey = -y2;
-            double lR = sqrt(ex * ex + ey * ey);
-            double ux = ex / lR, uy = ey / lR;
+            const double lR = cf[3], ux = cf[4], uy = cf[5];
             double rx0 = sx - x2, ry0 = sy - y2;
             double spx = rx0 * ux + ry0 * uy, spy = ux * ry0 - uy * rx0;
             auto hit = [&](double bx) {
@@ -779,6 +779,7 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
         const int q = (h + lane) % T.qcap;
         int fk = -1, kind = 0, g = -1, fs = 0, k = 0;  // kind: 0 window, 1 vertex event, 2 vertex-face item
         double b0 = 0, b1 = 0, sx = 0, sy = 0, sg = 0, L = 0, x2 = 0, y2 = 0;
+        double cf[6] = {0, 0, 0, 0, 0, 0};
         float2 rot = make_float2(0.f, 0.f);
         if (mine) {
             fk = T.qfk[q];
@@ -806,6 +807,8 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
                 L = geo[0];
                 x2 = geo[1];
                 y2 = geo[2];
+                if (kind == 0)
+                    for (int i = 0; i < 6; ++i) cf[i] = M.fchild[18 * g + 6 * k + i];
             }
         }
         // once every lane holds its record, the ring slots [h, h + nb) are free for this step's appends
@@ -815,9 +818,10 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
         bool live = false;
         if (mine) {
             if (kind == 0) {
-                live = keyOf(b0, b1, sx, sy, sg) <= C.rMax &&
-                       !filtered(T, T.fcv[3 * fs + k], T.fcv[3 * fs + (k + 1) % 3], L, b0, b1, sx, sy, sg);
-                if (live) propagate(C, fk, g, L, x2, y2, b0, b1, sx, sy, sg, rot, cP0, cP1, cP2);
+                const double r0 = endDist(b0, sx, sy), r1 = endDist(b1, sx, sy);
+                live = keyOf(b0, b1, sx, sy, sg, r0, r1) <= C.rMax &&
+                       !filtered(T, T.fcv[3 * fs + k], T.fcv[3 * fs + (k + 1) % 3], L, b0, b1, sg, r0, r1);
+                if (live) propagate(C, fk, g, L, x2, y2, b0, b1, sx, sy, sg, r0, r1, cf, rot, cP0, cP1, cP2);
             } else if (kind == 1) {
                 int vs = -fk - 1;
                 if (T.vdist[vs] == dbits(sg) && sg <= C.rMax) fanOutVertex(C, vs, sg);
@@ -834,7 +838,8 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
         ldsBarrier();
         if (live) {
             if (kind == 0 && T.fhead[fs] >= 0)
-                windowTargets<true, TT>(C, fs, g, k, L, x2, y2, b0, b1, sx, sy, sg, fk & 1, rot);
+                windowTargets<true, TT>(C, fs, g, k, L, x2, y2, b0, b1, sx, sy, sg, endDist(b0, sx, sy), endDist(b1, sx, sy), fk & 1,
+                                        rot);
             finishVertexCand(C, cP0);
             finishVertexCand(C, cP1);
             finishVertexCand(C, cP2);
@@ -1182,6 +1187,30 @@ __global__ __launch_bounds__(64) void patchKernel(GeoParams P, int32_t* list, in
     int32_t* out = list + (size_t)src * stride;
     for (int i = lane; i < n; i += blockDim.x) out[i] = fglob[i];
     if (lane == 0) count[src] = n;
+}
+
+// Frames of the two edges a window entering face g through edge k leaves by, per (face, edge): 6 doubles
+__global__ __launch_bounds__(256) void childFramesKernel(DeviceMesh M, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 3 g + k
+    if (i >= 3 * M.nF) return;
+    const double* geo = M.fgeo + 3 * i;
+    const double L = geo[0], x2 = geo[1], y2 = geo[2];
+    const double lL = sqrt(x2 * x2 + y2 * y2);
+    const double ex = L - x2, ey = -y2;
+    const double lR = sqrt(ex * ex + ey * ey);
+    double* o = out + 6 * (size_t)i;
+    o[0] = lL;
+    o[1] = x2 / lL;
+    o[2] = y2 / lL;
+    o[3] = lR;
+    o[4] = ex / lR;
+    o[5] = ey / lR;
+}
+
+hipError_t launchChildFrames(const DeviceMesh& mesh, double* out, hipStream_t stream) {
+    if (mesh.nF <= 0) return hipSuccess;
+    hipLaunchKernelGGL(childFramesKernel, dim3((3 * mesh.nF + 255) / 256), dim3(256), 0, stream, mesh, out);
+    return hipGetLastError();
 }
 
 hipError_t launchPatch(const GeoParams& P, int32_t* list, int32_t* count, int stride, hipStream_t stream) {
