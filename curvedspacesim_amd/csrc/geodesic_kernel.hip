@@ -247,12 +247,7 @@ struct VCand {  // one pending vertex candidate of a lane, carried from pass 1 t
 
 __device__ inline VCand noCand() { return VCand{-1, 0.0, make_float2(0.f, 0.f), false}; }
 
-template <class TT>
-__device__ inline void offerVertex(const TT& T, VCand& c, int slot, double d, float2 t) {
-    if (slot < 0) return;
-    if (c.slot == slot && !(d < c.d)) return;
-    unsigned long long old = atomicMin(&T.vdist[slot], dbits(d));
-    bool imp = d < bitsd(old);
+__device__ inline void offerVertexTail(VCand& c, int slot, double d, float2 t, bool imp) {
     if (c.slot == slot) {
         c.improved = c.improved || imp;
     } else {
@@ -262,6 +257,18 @@ __device__ inline void offerVertex(const TT& T, VCand& c, int slot, double d, fl
     c.d = d;
     c.t = t;
 }
+
+// Offers distance d (start tangent t) to a vertex slot: atomic minimum, remembered in the lane's candidate c for the
+// second pass.
+template <class TT>
+__device__ inline void offerVertex(const TT& T, VCand& c, int slot, double d, float2 t) {
+    if (slot < 0) return;
+    if (c.slot == slot && !(d < c.d)) return;
+    unsigned long long old = atomicMin(&T.vdist[slot], dbits(d));
+    bool imp = d < bitsd(old);
+    offerVertexTail(c, slot, d, t, imp);
+}
+
 
 // distance from the window's source (sx, sy) to the point b of its edge
 __device__ inline double endDist(double b, double sx, double sy) { return sqrt((b - sx) * (b - sx) + sy * sy); }
@@ -275,14 +282,17 @@ __device__ inline double keyOf(double b0, double b1, double sx, double sy, doubl
 
 // Xin-Wang filter against the edge's end vertices: the window is useless when one of its interval ends is reached
 // more cheaply through the vertex at the other end of the edge
-template <class TT>
-__device__ inline bool filtered(const TT& T, int vA, int vB, double L, double b0, double b1, double sg, double r0, double r1) {
-    double dA = bitsd(T.vdist[vA]), dB = bitsd(T.vdist[vB]);
+__device__ inline bool filteredBy(double dA, double dB, double L, double b0, double b1, double sg, double r0, double r1) {
     double toI1 = sg + r1;
     double toI0 = sg + r0;
     if (toI1 > dA + b1 + kFilterTol * toI1) return true;
     if (toI0 > dB + (L - b0) + kFilterTol * toI0) return true;
     return false;
+}
+
+template <class TT>
+__device__ inline bool filtered(const TT& T, int vA, int vB, double L, double b0, double b1, double sg, double r0, double r1) {
+    return filteredBy(bitsd(T.vdist[vA]), bitsd(T.vdist[vB]), L, b0, b1, sg, r0, r1);
 }
 
 template <class TT>
@@ -353,6 +363,31 @@ __device__ void addWindow(const Ctx<TT>& C, int fs, int k, double L, double b0, 
     pushItem(C, b0, b1, sx, sy, sg, fs * 8 + k * 2 + (rooted ? 1 : 0), rot);
 }
 
+// A child window of a window being propagated: as addWindow, with its edge's end vertices (slots vA, vB) and the
+// distances the lane last saw for them (dA, dB; lowered here by the lane's own offers) given by the caller. Any
+// distances that a path reached are valid bounds for the filter.
+template <class TT>
+__device__ void addChild(const Ctx<TT>& C, int fs, int k, double L, double b0, double b1, double sx, double sy, double sg, bool rooted,
+                         float2 rot, int vA, double& dA, int vB, double& dB, VCand& cA, VCand& cB) {
+    if (!(b1 > b0)) return;
+    const double r0 = endDist(b0, sx, sy), r1 = endDist(b1, sx, sy);
+    if (keyOf(b0, b1, sx, sy, sg, r0, r1) > C.rMax) return;
+    const TT& T = C.T;
+    double tol = kEdgeTol * L;
+    if (b0 <= tol) {
+        const double d = sg + (b0 == 0.0 ? r0 : endDist(0.0, sx, sy));
+        offerVertex(T, cA, vA, d, startTangent(rooted, rot, -sx, -sy));
+        dA = fmin(dA, d);
+    }
+    if (b1 >= L - tol) {
+        const double d = sg + (b1 == L ? r1 : endDist(L, sx, sy));
+        offerVertex(T, cB, vB, d, startTangent(rooted, rot, L - sx, -sy));
+        dB = fmin(dB, d);
+    }
+    if (filteredBy(dA, dB, L, b0, b1, sg, r0, r1)) return;
+    pushItem(C, b0, b1, sx, sy, sg, fs * 8 + k * 2 + (rooted ? 1 : 0), rot);
+}
+
 template <class TT>
 __device__ inline void pushVertexEvent(const Ctx<TT>& C, int vslot, double d) {
     pushItem(C, 0.0, 0.0, 0.0, 0.0, d, -vslot - 1, make_float2(0.f, 0.f));
@@ -414,20 +449,33 @@ __device__ void windowTargets(const Ctx<TT>& C, int fs, int g, int k, double L, 
 }
 
 // Window propagation across the face it enters (pass 1): candidate for the opposite corner, up to two children.
+// The corners of the face a window enters, in the order k, k+1, k+2 of its entry edge k: vertex slots, the distances
+// the lane read for them, and the neighbours across edges k+2 and k+1 (the edges the children leave by). A consistently
+// oriented mesh runs the shared edge the other way in the neighbour, so the left child's edge goes from corner k to
+// corner k+2 and the right child's from corner k+2 to corner k+1.
+struct Corners {
+    int vA, vB, vC, nbL, nbR;
+    double dA, dB, dC;
+};
+
+// Window propagation across the face it enters (pass 1): candidate for the opposite corner, up to two children.
 template <class TT>
 __device__ void propagate(const Ctx<TT>& C, int fk, int g, double L, double x2, double y2, double b0, double b1, double sx, double sy,
-                          double sg, double r0, double r1, const double* cf, float2 rot, VCand& cP0, VCand& cP1, VCand& cP2) {
+                          double sg, double r0, double r1, const double* cf, float2 rot, Corners& K, VCand& cP0, VCand& cP1,
+                          VCand& cP2) {
     const TT& T = C.T;
     int fs = fk >> 3, k = (fk >> 1) & 3;
     bool rooted = fk & 1;
-    int vC = T.fcv[3 * fs + (k + 2) % 3];
     windowTargets<false, TT>(C, fs, g, k, L, x2, y2, b0, b1, sx, sy, sg, r0, r1, rooted, rot);
     double xs = sx + (x2 - sx) * (-sy) / (y2 - sy);  // where the ray through the opposite corner crosses the entry edge
     double tolE = kEdgeTol * L;
-    if (xs >= b0 - tolE && xs <= b1 + tolE)
-        offerVertex(T, cP2, vC, sg + sqrt((x2 - sx) * (x2 - sx) + (y2 - sy) * (y2 - sy)), startTangent(rooted, rot, x2 - sx, y2 - sy));
+    if (xs >= b0 - tolE && xs <= b1 + tolE) {
+        const double d = sg + sqrt((x2 - sx) * (x2 - sx) + (y2 - sy) * (y2 - sy));
+        offerVertex(T, cP2, K.vC, d, startTangent(rooted, rot, x2 - sx, y2 - sy));
+        K.dC = fmin(K.dC, d);
+    }
     if (xs > b0) {  // left child: edge corner k -> corner k+2, entered from the neighbour across local edge k+2
-        int nb = T.fnb[3 * fs + (k + 2) % 3];
+        int nb = K.nbL;
         if (nb >= 0) {
             const double lL = cf[0], ux = cf[1], uy = cf[2];
             double spx = sx * ux + sy * uy, spy = ux * sy - uy * sx;
@@ -440,11 +488,11 @@ __device__ void propagate(const Ctx<TT>& C, int fk, int g, double L, double x2, 
             double t0 = (b0 <= 0) ? 0.0 : hit(b0);
             double t1 = (xs <= b1) ? lL : hit(b1);
             float2 r2 = rooted ? make_float2((float)(rot.x * ux - rot.y * uy), (float)(rot.y * ux + rot.x * uy)) : rot;
-            addWindow(C, nb >> 2, nb & 3, lL, t0, t1, spx, fmin(spy, -1e-14 * lL), sg, rooted, r2, cP0, cP2);
+            addChild(C, nb >> 2, nb & 3, lL, t0, t1, spx, fmin(spy, -1e-14 * lL), sg, rooted, r2, K.vA, K.dA, K.vC, K.dC, cP0, cP2);
         }
     }
     if (xs < b1) {  // right child: edge corner k+2 -> corner k+1, entered from the neighbour across local edge k+1
-        int nb = T.fnb[3 * fs + (k + 1) % 3];
+        int nb = K.nbR;
         if (nb >= 0) {
             const double lR = cf[3], ux = cf[4], uy = cf[5];
             double rx0 = sx - x2, ry0 = sy - y2;
@@ -458,7 +506,7 @@ __device__ void propagate(const Ctx<TT>& C, int fk, int g, double L, double x2, 
             double t0 = (xs >= b0) ? 0.0 : hit(b0);
             double t1 = (b1 >= L) ? lR : hit(b1);
             float2 r2 = rooted ? make_float2((float)(rot.x * ux - rot.y * uy), (float)(rot.y * ux + rot.x * uy)) : rot;
-            addWindow(C, nb >> 2, nb & 3, lR, t0, t1, spx, fmin(spy, -1e-14 * lR), sg, rooted, r2, cP2, cP1);
+            addChild(C, nb >> 2, nb & 3, lR, t0, t1, spx, fmin(spy, -1e-14 * lR), sg, rooted, r2, K.vC, K.dC, K.vB, K.dB, cP2, cP1);
         }
     }
 }
@@ -819,9 +867,18 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
         if (mine) {
             if (kind == 0) {
                 const double r0 = endDist(b0, sx, sy), r1 = endDist(b1, sx, sy);
-                live = keyOf(b0, b1, sx, sy, sg, r0, r1) <= C.rMax &&
-                       !filtered(T, T.fcv[3 * fs + k], T.fcv[3 * fs + (k + 1) % 3], L, b0, b1, sg, r0, r1);
-                if (live) propagate(C, fk, g, L, x2, y2, b0, b1, sx, sy, sg, r0, r1, cf, rot, cP0, cP1, cP2);
+                const int k1 = k == 2 ? 0 : k + 1, k2 = k == 0 ? 2 : k - 1;
+                Corners K;
+                K.vA = T.fcv[3 * fs + k];
+                K.vB = T.fcv[3 * fs + k1];
+                K.vC = T.fcv[3 * fs + k2];
+                K.nbL = T.fnb[3 * fs + k2];
+                K.nbR = T.fnb[3 * fs + k1];
+                K.dA = bitsd(T.vdist[K.vA]);
+                K.dB = bitsd(T.vdist[K.vB]);
+                K.dC = bitsd(T.vdist[K.vC]);
+                live = keyOf(b0, b1, sx, sy, sg, r0, r1) <= C.rMax && !filteredBy(K.dA, K.dB, L, b0, b1, sg, r0, r1);
+                if (live) propagate(C, fk, g, L, x2, y2, b0, b1, sx, sy, sg, r0, r1, cf, rot, K, cP0, cP1, cP2);
             } else if (kind == 1) {
                 int vs = -fk - 1;
                 if (T.vdist[vs] == dbits(sg) && sg <= C.rMax) fanOutVertex(C, vs, sg);
