@@ -8,7 +8,7 @@ namespace csg {
 
 // Capacities of the LDS variant (one source per workgroup of kLdsThreads lanes, under 80 KB so that two workgroups
 // share a CU): patch faces and vertices, records of the work-item ring, targets per source, and the slots of the
-// setup-time face and vertex hashes. A source that exceeds the first, second or fourth, or its spill stack, is solved
+// setup-time face and vertex hashes. A source that exceeds the first, second or fourth, or its spill queue, is solved
 // again by the scratch variant.
 constexpr int kLdsFaceCap = 640;
 constexpr int kLdsVertCap = 384;

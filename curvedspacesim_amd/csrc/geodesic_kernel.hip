@@ -1,15 +1,15 @@
 // Batched exact polyhedral geodesics + pair forces on gfx950.
 //
-// One workgroup (four 64-wide wavefronts) solves one source particle; two workgroups share a CU, so that one's waves
-// run while the other's wait at a barrier or on memory:
+// One workgroup (four or eight 64-wide wavefronts) solves one source particle; two to four workgroups share a CU, so
+// that one's waves run while another's wait at a barrier or on memory:
 //   1. patch   : the face set of the reference's submesh rule (src/utility/submesher.cpp:55-147), prepared by the
 //                patch kernel, is numbered into compact LDS slots through setup-time hash tables (global id -> slot);
 //                every face slot caches its corners' vertex slots and its neighbours' face slots, so the propagation
 //                touches HBM only for face geometry and coordinates.
 //   2. windows : exact window (interval) propagation in FP64 (the method behind CGAL's Surface_mesh_shortest_path,
 //                src/utility/meshUtilities.h:44-48), up to 256 work items per step from an LDS ring, one per lane;
-//                items that do not fit the ring wait in a per-source HBM spill stack and are moved back when the ring
-//                drains (the result does not depend on the order in which windows are processed).
+//                items that do not fit the ring wait in a per-source HBM spill queue, oldest first, and fill the free
+//                lanes of later batches (the result does not depend on the order in which windows are processed).
 //                Vertex distance bounds are LDS 64-bit atomic minima on the IEEE bit pattern (distances are
 //                non-negative), and a window is dropped when those bounds show it cannot carry a shortest path
 //                anywhere on its interval (Xin & Wang's filter). Saddle vertices (angle sum >= 2 pi) and vertices on
@@ -83,7 +83,7 @@ struct Tables {
 
 struct Shared {
     int head, tail, nFaces, nVerts, overflow, nCur, nNext, goals, nSteps, nItems;
-    int spillTop;  // records in the source's HBM spill stack
+    int spillHead, spillTail;  // the source's HBM spill queue: records [spillHead, spillTail), ring positions mod spillCap
     int srcSlot;   // face slot of the source face
     double S[3], t1[3], t2[3];
     double red[kMaxWaves][4];
@@ -303,10 +303,10 @@ struct Ctx {
     int qBase;  // head at the start of the step: appends stay below qBase + qcap, so live records are never overwritten
     double rMax;
     int tb;     // first pair index of the source
-    SpillRecord* spill;  // this source's HBM spill stack (nullptr: none)
+    SpillRecord* spill;  // this source's HBM spill queue (nullptr: none)
 };
 
-// Appends one work item: to the LDS ring while it has room, else to the source's HBM spill stack. A ring position at
+// Appends one work item: to the LDS ring while it has room, else to the source's HBM spill queue. A ring position at
 // or past qBase + qcap is handed back at once, so the positions below it are handed out exactly once.
 template <class TT>
 __device__ void pushItem(const Ctx<TT>& C, double b0, double b1, double sx, double sy, double sg, int32_t fk, float2 t) {
@@ -325,9 +325,10 @@ __device__ void pushItem(const Ctx<TT>& C, double b0, double b1, double sx, doub
     }
     atomicSub(&C.sh.tail, 1);
     if (C.spill) {
-        int sp = atomicAdd(&C.sh.spillTop, 1);
-        if (sp >= C.P.spillCap) {
-            atomicSub(&C.sh.spillTop, 1);  // as for the ring: entries past the capacity are handed back
+        // spillHead only moves between full barriers, never while items are appended
+        int sp = atomicAdd(&C.sh.spillTail, 1);
+        if (sp - C.sh.spillHead >= C.P.spillCap) {
+            atomicSub(&C.sh.spillTail, 1);  // as for the ring: positions past the capacity are handed back
         } else {
             SpillRecord r;
             r.b0 = b0;
@@ -339,7 +340,7 @@ __device__ void pushItem(const Ctx<TT>& C, double b0, double b1, double sx, doub
             r.tx = t.x;
             r.ty = t.y;
             r.pad = 0;
-            C.spill[sp] = r;
+            C.spill[sp % C.P.spillCap] = r;
             return;
         }
     }
@@ -792,7 +793,7 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
     int h = 0;  // every lane advances its own copy of the head
     for (int step = 0;; ++step) {
         const int t = sh.tail;
-        const int sp = sh.spillTop;
+        const int sp = sh.spillTail - sh.spillHead;  // spilled records waiting
         const bool ovf = sh.overflow != 0;
         ldsBarrier();  // every wave has read the tail and the flags before any wave appends or sets one
         if (ovf || (h >= t && sp == 0)) break;
@@ -800,11 +801,13 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
             if (lane == 0) atomicOr(&sh.overflow, OVF_STEPS);
             break;
         }
-        if (h >= t) {  // the ring is empty: the newest spilled records (at most a ring's worth) move back into it
+        const int room = min((int)blockDim.x, T.qcap) - (t - h);  // free lanes of the next batch (and free ring slots)
+        if (sp > 0 && room > 0) {  // the oldest spilled records fill the batch before it runs
             __syncthreads();  // a full barrier: every wave's spill stores are visible before they are read
-            const int m = min(sp, T.qcap);
+            const int m = min(sp, room);
+            const int s0 = sh.spillHead;
             for (int i = lane; i < m; i += blockDim.x) {
-                const SpillRecord r = spill[sp - m + i];
+                const SpillRecord r = spill[(s0 + i) % P.spillCap];
                 const int q = (t + i) % T.qcap;
                 T.qb0[q] = r.b0;
                 T.qb1[q] = r.b1;
@@ -817,7 +820,7 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
             __syncthreads();
             if (lane == 0) {
                 sh.tail = t + m;
-                sh.spillTop = sp - m;
+                sh.spillHead = s0 + m;
             }
             __syncthreads();
             continue;
@@ -921,7 +924,7 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
     for (int i = lane; i <= T.vhmask; i += blockDim.x) T.vhkey[i] = -1;
     for (int i = lane; i < T.vcap; i += blockDim.x) T.vfo[i] = 0;
     if (lane == 0) {
-        sh.head = sh.tail = sh.nFaces = sh.nVerts = sh.nSteps = sh.nItems = sh.spillTop = 0;
+        sh.head = sh.tail = sh.nFaces = sh.nVerts = sh.nSteps = sh.nItems = sh.spillHead = sh.spillTail = 0;
         sh.srcSlot = -1;
         sh.overflow = nT > T.tcap ? OVF_TARGETS : 0;
         sh.nCur = sh.nNext = sh.goals = 0;
