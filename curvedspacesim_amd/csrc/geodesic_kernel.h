@@ -83,7 +83,7 @@ ScratchLayout makeScratchLayout(long faces, long targets);
 hipError_t launchPatch(const GeoParams& P, int32_t* list, int32_t* count, int stride, hipStream_t stream);
 // LDS variants: 0 = 256 lanes (4 wavefronts) per source and 1 = 512 lanes (8 wavefronts), both two workgroups per CU
 // with the capacities above; 2 = 256 lanes, three workgroups per CU with 576 faces, 352 vertices, a 384-record ring
-// and 224 targets; 3 = 256 lanes, four workgroups per CU (40 KB each) with 576 faces, 320 vertices, a 240-record ring
+// and 224 targets; 3 = 256 lanes, four workgroups per CU (40 KB each) with 576 faces, 320 vertices, a 280-record ring
 // and 128 targets
 constexpr int kLdsVariants = CSG_LDS_VARIANTS;
 hipError_t launchGeodesicLds(const GeoParams& P, int variant, hipStream_t stream);

@@ -46,8 +46,8 @@ __device__ inline unsigned long long dbits(double d) { return (unsigned long lon
 __device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 __device__ inline uint32_t hashOf(uint32_t key) { return (key * 2654435761u) >> 7; }
 
-// Tables of one source. LDS variant: every pointer refers to __shared__ storage (the setup hashes share the FIFO's
-// storage, which is idle until the propagation starts); scratch variant: to a slice of a global buffer. Faces and
+// Tables of one source. LDS variant: every pointer refers to __shared__ storage (the setup hashes and the vertex ids
+// share the FIFO's storage, which is idle until the propagation starts); scratch variant: to a slice of a global buffer. Faces and
 // vertices are numbered 0, 1, ... in insertion order (compact slots); hash capacities are powers of two.
 template <typename IT>
 struct Tables {
@@ -59,7 +59,7 @@ struct Tables {
     IT* fcv;         // 3 per face slot: corner vertex slots
     IT* fnb;         // 3 per face slot: neighbour face slot * 4 + its local edge index, or -1 outside the patch
     IT* fhead;       // per face slot: first target lying in that face, or -1
-    int32_t* vglob;  // per vertex slot: global vertex id
+    int32_t* vglob;  // per vertex slot: global vertex id (setup only)
     unsigned long long* vdist;
     float2* vtan;    // start tangent of the best path to the vertex (source-face frame)
     uint8_t* vflg;   // 1: may act as a pseudo-source
@@ -86,7 +86,6 @@ struct Shared {
     int spillHead, spillTail;  // the source's HBM spill queue: records [spillHead, spillTail), ring positions mod spillCap
     int srcSlot;   // face slot of the source face
     double S[3], t1[3], t2[3];
-    double red[kMaxWaves][4];
     int wsum[kMaxWaves];
 };
 
@@ -1033,22 +1032,23 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
     F1 = wsum(F1);
     F2 = wsum(F2);
     E = wsum(E);
-    if (blockDim.x > 64) {  // combine the wavefronts' partial sums in fixed order
+    if (blockDim.x > 64) {  // combine the wavefronts' partial sums in fixed order, through the idle ring storage
+        double* red = T.qb0;
         __syncthreads();
         if ((lane & 63) == 0) {
-            sh.red[lane >> 6][0] = F0;
-            sh.red[lane >> 6][1] = F1;
-            sh.red[lane >> 6][2] = F2;
-            sh.red[lane >> 6][3] = E;
+            red[4 * (lane >> 6)] = F0;
+            red[4 * (lane >> 6) + 1] = F1;
+            red[4 * (lane >> 6) + 2] = F2;
+            red[4 * (lane >> 6) + 3] = E;
         }
         __syncthreads();
         if (lane == 0) {
             F0 = F1 = F2 = E = 0;
             for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-                F0 += sh.red[w][0];
-                F1 += sh.red[w][1];
-                F2 += sh.red[w][2];
-                E += sh.red[w][3];
+                F0 += red[4 * w];
+                F1 += red[4 * w + 1];
+                F2 += red[4 * w + 2];
+                E += red[4 * w + 3];
             }
         }
     }
@@ -1091,18 +1091,19 @@ int gridForSources(const GeoParams& P) { return P.blockOrder ? kXcds * P.blockCh
 
 // LDS variant: tables in __shared__ memory, one source per workgroup of NT lanes, WGS workgroups per CU (each within
 // 160 KB / WGS of LDS). FC faces, VC vertices, QC ring records, TC targets; HF and HV face and vertex hash slots, which
-// share the ring's storage; kEnd: the variant that also reports end tangents.
+// share the ring's storage with the vertex ids during setup and with the force reduction afterwards; kEnd: the variant
+// that also reports end tangents.
 template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, int WGS, bool kEnd>
 __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P) {
     constexpr int kRingBytes = QC * (5 * 8 + 4 + 8);
-    static_assert(HF * 6 + HV * 6 <= kRingBytes, "setup hashes live in the ring's storage");
+    static_assert(HF * 6 + HV * 6 + VC * 4 <= kRingBytes, "setup hashes and vertex ids live in the ring's storage");
+    static_assert(QC >= 4 * kMaxWaves, "the epilogue's reduction uses the ring's storage");
     static_assert(QC % 2 == 0 && (HF & (HF - 1)) == 0 && (HV & (HV - 1)) == 0, "alignment and power-of-two hashes");
     static_assert(4 * FC <= 32767 && VC <= 32767 && TC <= 32767, "slots must fit int16");
     static_assert(NT % 64 == 0 && NT / 64 <= kMaxWaves, "whole wavefronts");
     __shared__ __attribute__((aligned(16))) unsigned char ring[kRingBytes];
     __shared__ int32_t fglob[FC];
     __shared__ int16_t fcv[3 * FC], fnb[3 * FC], fhead[FC], vfl[3 * FC];
-    __shared__ int32_t vglob[VC];
     __shared__ unsigned long long vdist[VC];
     __shared__ float2 vtan[VC];
     __shared__ uint8_t vflg[VC];
@@ -1125,11 +1126,11 @@ __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P) {
     T.hslot = (int16_t*)(T.hkey + HF);
     T.vhkey = (int32_t*)(T.hslot + HF);
     T.vhslot = (int16_t*)(T.vhkey + HV);
+    T.vglob = (int32_t*)(T.vhslot + HV);  // read during setup only
     T.fglob = fglob;
     T.fcv = fcv;
     T.fnb = fnb;
     T.fhead = fhead;
-    T.vglob = vglob;
     T.vdist = vdist;
     T.vtan = vtan;
     T.vflg = vflg;
@@ -1319,7 +1320,7 @@ hipError_t launchGeodesicLds(const GeoParams& P, int variant, hipStream_t stream
     else if (variant == 2)  // smaller tables and ring, three workgroups per CU
         launchLdsVariant<576, 352, 384, 224, 1024, 512, 256, 3>(P, stream);
     else if (variant == 3)  // four workgroups per CU: 40 KB each
-        launchLdsVariant<576, 320, 240, 128, 1024, 512, 256, 4>(P, stream);
+        launchLdsVariant<576, 320, 280, 128, 1024, 512, 256, 4>(P, stream);
     else
         launchLdsVariant<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsFaceHash, kLdsVertHash, 256, 2>(P, stream);
     return hipGetLastError();

@@ -145,7 +145,7 @@ csg_status csg_system_patch_stats(csg_ctx* ctx, int64_t* sums, int32_t* maxima);
 csg_status csg_system_fallback_counts(csg_ctx* ctx, int32_t* reasons);
 /* variants of the LDS geodesic kernel: 0 and 1: 256 and 512 lanes per source, two workgroups per CU; 2: 256 lanes, three
  * workgroups per CU with smaller tables and work-item ring; 3: 256 lanes, four workgroups per CU (40 KB of LDS each:
- * 320 patch vertices, 128 targets, a 240-record ring). Sources that do not fit a variant's tables are solved by the
+ * 320 patch vertices, 128 targets, a 280-record ring). Sources that do not fit a variant's tables are solved by the
  * HBM-table kernel. */
 #define CSG_LDS_VARIANTS 4
 /* autotuner (no reference counterpart): times the force computation of the current state with each LDS kernel variant
