@@ -24,6 +24,10 @@ struct DeviceMesh {
     // {lL, uxL, uyL, lR, uxR, uyR}, the length and unit direction (in the edge-k frame) of corner k -> corner k+2 and
     // of corner k+2 -> corner k+1
     const double* __restrict__ fchild;
+    // 9 nF each, for the submesh search: a face's corner coordinates, and per local edge k the coordinates of the
+    // neighbour's corner opposite the shared edge (zero on a border edge)
+    const double* __restrict__ fcorn;
+    const double* __restrict__ fnbv;
 };
 
 // pair potential, reference conventions (src/forces/harmonicRepulsion.cpp, src/forces/gaussianRepulsion.cpp)

@@ -50,7 +50,7 @@ struct csg_ctx {
     // mesh
     int nV = 0, nF = 0;
     double area = 0, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    DevBuf vx, fv, fadj, fgeo, fnrm, vflag, vfoff, vflist, curv, fchild;
+    DevBuf vx, fv, fadj, fgeo, fnrm, vflag, vfoff, vflist, curv, fchild, fcorn, fnbv;
     csg::DeviceMesh dm{};
     bool haveMesh = false;
     int submeshing = 0;
@@ -65,7 +65,7 @@ struct csg_ctx {
     csg::SystemState* sys = nullptr;
     ~csg_ctx() {
         if (sys) csg::destroySystem(sys);
-        DevBuf* all[] = {&vx, &fv, &fadj, &fgeo, &fnrm, &vflag, &vfoff, &vflist, &curv, &fchild, &sX, &sY, &sOut, &sPart, &sGrad, &sParams, &sM, &sV, &pface, &pbary, &srcIdx, &toff, &tidx, &prad,
+        DevBuf* all[] = {&vx, &fv, &fadj, &fgeo, &fnrm, &vflag, &vfoff, &vflist, &curv, &fchild, &fcorn, &fnbv, &sX, &sY, &sOut, &sPart, &sGrad, &sParams, &sM, &sV, &pface, &pbary, &srcIdx, &toff, &tidx, &prad,
                          &pdist, &pst, &pet, &pflags, &pforces, &penergy, &status, &list, &plist, &pcount, &spill, &tdisp, &tvecs, &tstat, &scratch};
         for (DevBuf* b : all) b->release();
         if (stream) hipStreamDestroy(stream);

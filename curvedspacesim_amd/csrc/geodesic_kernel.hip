@@ -59,6 +59,7 @@ struct Tables {
     IT* fcv;         // 3 per face slot: corner vertex slots
     IT* fnb;         // 3 per face slot: neighbour face slot * 4 + its local edge index, or -1 outside the patch
     IT* fhead;       // per face slot: first target lying in that face, or -1
+    int32_t* thead;  // setup only: the same lists' heads as 32-bit words, built with atomic exchanges
     int32_t* vglob;  // per vertex slot: global vertex id (setup only)
     unsigned long long* vdist;
     float2* vtan;    // start tangent of the best path to the vertex (source-face frame)
@@ -617,20 +618,27 @@ __device__ void patchSearch(const GeoParams& P, const TT& T, Shared& sh, int* cu
             const int nc = sh.nCur;
             if (nc == 0) break;
             for (int i = lane; i < nc; i += blockDim.x) {
-                int f = cur[i];
+                const int f = cur[i];
+                // the neighbour across edge k has corners k and k+1 of f and one more: one load level per face
+                int adj[3];
+                bool in[3], inOpp[3];
                 for (int k = 0; k < 3; ++k) {
-                    int a = M.fadj[3 * f + k];
+                    adj[k] = M.fadj[3 * f + k];
+                    const double* p = M.fcorn + 9 * f + 3 * k;
+                    const double* q = M.fnbv + 9 * f + 3 * k;
+                    double dx = p[0] - sh.S[0], dy = p[1] - sh.S[1], dz = p[2] - sh.S[2];
+                    in[k] = !(dx * dx + dy * dy + dz * dz > r2);
+                    dx = q[0] - sh.S[0];
+                    dy = q[1] - sh.S[1];
+                    dz = q[2] - sh.S[2];
+                    inOpp[k] = !(dx * dx + dy * dy + dz * dz > r2);
+                }
+                for (int k = 0; k < 3; ++k) {
+                    int a = adj[k];
                     if (a < 0) continue;
                     int g = a >> 2;
+                    if (!(in[k] || in[k == 2 ? 0 : k + 1] || inOpp[k])) continue;  // every corner outside the radius
                     if (faceKnown(T, g)) continue;
-                    bool outside = true;
-                    for (int j = 0; j < 3; ++j) {
-                        double p[3];
-                        loadV(M, M.fv[3 * g + j], p);
-                        double dx = p[0] - sh.S[0], dy = p[1] - sh.S[1], dz = p[2] - sh.S[2];
-                        if (!(dx * dx + dy * dy + dz * dz > r2)) outside = false;
-                    }
-                    if (outside) continue;
                     bool fresh;
                     faceInsert(T, sh, g, fresh);
                     if (fresh) {
@@ -707,7 +715,7 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int* cur
             const int ns = a < 0 ? -1 : faceLookup(T, a >> 2);
             T.fnb[3 * s + k] = ns < 0 ? -1 : ns * 4 + (a & 3);
         }
-        T.fhead[s] = -1;
+        T.thead[s] = -1;
     }
     for (int t = lane; t < nT; t += blockDim.x) {
         const int pj = P.tidx[tb + t];
@@ -731,16 +739,16 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int* cur
         T.vtan[s] = make_float2(0.f, 0.f);
     }
     blockExclusiveScan(T.vfo, nV, sh);
-    // E. the vertex -> face lists; targets linked to their faces by one lane (fixed order)
+    // E. the vertex -> face lists; each target pushed onto its face's list (the order within a list does not matter:
+    // every window tests all targets of its face)
     for (int s = lane; s < nF; s += blockDim.x)
         for (int k = 0; k < 3; ++k) T.vfl[atomicAdd(&T.vfo[T.fcv[3 * s + k]], 1)] = s * 4 + k;
-    if (lane == 0)
-        for (int t = nT - 1; t >= 0; --t) {
-            const int s = T.tslot[t];
-            if (s < 0) continue;
-            T.tnext[t] = T.fhead[s];
-            T.fhead[s] = t;
-        }
+    for (int t = lane; t < nT; t += blockDim.x) {
+        const int s = T.tslot[t];
+        if (s >= 0) T.tnext[t] = atomicExch(&T.thead[s], t);
+    }
+    __syncthreads();
+    for (int s = lane; s < nF; s += blockDim.x) T.fhead[s] = T.thead[s];
     __syncthreads();
     return true;
 }
@@ -1096,7 +1104,7 @@ int gridForSources(const GeoParams& P) { return P.blockOrder ? kXcds * P.blockCh
 template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, int WGS, bool kEnd>
 __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P) {
     constexpr int kRingBytes = QC * (5 * 8 + 4 + 8);
-    static_assert(HF * 6 + HV * 6 + VC * 4 <= kRingBytes, "setup hashes and vertex ids live in the ring's storage");
+    static_assert(HF * 6 + HV * 6 + VC * 4 + FC * 4 <= kRingBytes, "setup hashes, vertex ids and list heads live in the ring's storage");
     static_assert(QC >= 4 * kMaxWaves, "the epilogue's reduction uses the ring's storage");
     static_assert(QC % 2 == 0 && (HF & (HF - 1)) == 0 && (HV & (HV - 1)) == 0, "alignment and power-of-two hashes");
     static_assert(4 * FC <= 32767 && VC <= 32767 && TC <= 32767, "slots must fit int16");
@@ -1127,6 +1135,7 @@ __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P) {
     T.vhkey = (int32_t*)(T.hslot + HF);
     T.vhslot = (int16_t*)(T.vhkey + HV);
     T.vglob = (int32_t*)(T.vhslot + HV);  // read during setup only
+    T.thead = T.vglob + VC;               // written during setup only
     T.fglob = fglob;
     T.fcv = fcv;
     T.fnb = fnb;
@@ -1171,6 +1180,7 @@ __global__ __launch_bounds__(64) void geodesicForceScratch(GeoParams P, ScratchL
     T.vhslot = (int32_t*)take(4 * (size_t)L.hvcap);
     T.fglob = (int32_t*)take(4 * (size_t)L.fcap);
     T.fhead = (int32_t*)take(4 * (size_t)L.fcap);
+    T.thead = (int32_t*)take(4 * (size_t)L.fcap);
     T.fcv = (int32_t*)take(12 * (size_t)L.fcap);
     T.fnb = (int32_t*)take(12 * (size_t)L.fcap);
     T.vfl = (int32_t*)take(12 * (size_t)L.fcap);
@@ -1297,7 +1307,7 @@ ScratchLayout makeScratchLayout(long faces, long targets) {
     auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t F = L.fcap, V = L.vcap, Q = L.qcap, Tn = L.tcap, HF = L.hfcap, HV = L.hvcap;
     L.bytesPerBlock = 2 * r(4 * HF) + 2 * r(4 * HV)                        // hashes
-                      + 2 * r(4 * F) + 3 * r(12 * F)                       // fglob, fhead; fcv, fnb, vfl
+                      + 3 * r(4 * F) + 3 * r(12 * F)                       // fglob, fhead, thead; fcv, fnb, vfl
                       + r(4 * V) + r(V) + r(4 * V) + 2 * r(8 * V)          // vglob, vflg, vfo; vdist, vtan
                       + 5 * r(8 * Q) + r(4 * Q) + r(8 * Q)                 // ring
                       + 2 * r(4 * Tn) + r(24 * Tn) + 2 * r(8 * Tn) + r(16 * Tn)  // targets
