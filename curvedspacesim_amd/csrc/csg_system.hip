@@ -36,7 +36,7 @@ struct SystemState {
     DevBuf face, bary, vel, force, mass, energy;
     DevBuf xyz, sxyz, key, cellCount, cellStart, cellFill, order;
     DevBuf cnt, off, toff, cand, maxd, rad;
-    DevBuf disp, tstatus, gstatus, partial, counters, ke, chain, pstat, plist, pcount, spill;
+    DevBuf disp, tstatus, gstatus, partial, counters, ke, chain, pstat, plist, pcount, spill, grab;
     DevBuf scratch;
     // learned screen: features and logits of the candidate pairs, kept-pair lists; recorded pair distances
     DevBuf sfeat, slogit, scnt, soff, scand, sparams, pdist;
@@ -96,7 +96,7 @@ void destroySystem(SystemState* s) {
     if (s->graph) hipGraphExecDestroy(s->graph);
     DevBuf* all[] = {&s->face, &s->bary, &s->vel, &s->force, &s->mass, &s->energy, &s->xyz, &s->sxyz, &s->key, &s->cellCount, &s->cellStart,
                      &s->cellFill, &s->order, &s->cnt, &s->off, &s->toff, &s->cand, &s->maxd, &s->rad, &s->disp, &s->tstatus,
-                     &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->spill, &s->scratch,
+                     &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->spill, &s->grab, &s->scratch,
                      &s->sfeat, &s->slogit, &s->scnt, &s->soff, &s->scand, &s->sparams, &s->pdist};
     for (DevBuf* b : all) b->release();
     for (auto& m : s->marks) {
@@ -553,7 +553,10 @@ static csg_status sysForces(csg_ctx* c, SystemState* S) {
         }
         {
             MarkScope mark(S, c->stream, MARK_GEODESIC_LDS);
+            G.grab = S->grab.as<int>();  // dynamic distribution over the XCDs' chunks; counters cleared by a kernel
+            hipLaunchKernelGGL(kFillInt, dim3(1), dim3(kBlock), 0, c->stream, G.grab, 0, (long)kXcds);
             HIPCHECK(c, launchGeodesicLds(G, S->ldsVariant, c->stream));
+            G.grab = nullptr;
         }
         hipLaunchKernelGGL(kCountReasons, dim3(blocksFor(S->N)), dim3(kBlock), 0, c->stream, S->N, S->gstatus.as<int>(), S->counters.as<int>());
         MarkScope mark(S, c->stream, MARK_GEODESIC_HBM);
@@ -716,6 +719,7 @@ csg_status csg_system_init(csg_ctx* c, int32_t N, const int32_t* face, const dou
     HIPCHECK(c, S->plist.ensure(4 * (size_t)kPatchListCap * N));
     HIPCHECK(c, S->pcount.ensure(4 * (size_t)N));
     HIPCHECK(c, S->spill.ensure(sizeof(SpillRecord) * (size_t)kSpillCap * N));
+    HIPCHECK(c, S->grab.ensure(sizeof(int) * kXcds));
     HIPCHECK(c, S->partial.ensure(8 * (size_t)blocksFor(N)));
     HIPCHECK(c, S->counters.ensure(4 * CNT_N));
     HIPCHECK(c, S->ke.ensure(16));

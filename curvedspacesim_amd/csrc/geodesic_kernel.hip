@@ -85,6 +85,7 @@ struct Tables {
 struct Shared {
     int head, tail, nFaces, nVerts, overflow, nCur, nNext, goals, nSteps, nItems;
     int spillHead, spillTail;  // the source's HBM spill queue: records [spillHead, spillTail), ring positions mod spillCap
+    int next;                  // the source the workgroup takes next (dynamic distribution)
     int srcSlot;   // face slot of the source face
     double S[3], t1[3], t2[3];
     int wsum[kMaxWaves];
@@ -1097,6 +1098,35 @@ __device__ inline int sourceOfBlock(const GeoParams& P) {
 
 int gridForSources(const GeoParams& P) { return P.blockOrder ? kXcds * P.blockChunk : P.nSrc; }
 
+// Dynamic distribution: the next source for a workgroup on XCD x, or -1 when every chunk is taken. Chunk c holds
+// positions [c * chunk, (c + 1) * chunk) of the order; each position is handed out once by its chunk's counter.
+__device__ int grabSource(const GeoParams& P, int x) {
+    const int n = P.nSrc, chunk = (n + kXcds - 1) / kXcds;
+    for (int d = 0; d < kXcds; ++d) {
+        const int c = (x + d) % kXcds;
+        const int lo = c * chunk, len = min(chunk, n - lo);
+        if (len <= 0) continue;
+        if (__hip_atomic_load(&P.grab[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= len) continue;
+        const int j = atomicAdd(&P.grab[c], 1);
+        if (j < len) return P.blockOrder ? P.blockOrder[lo + j] : lo + j;
+    }
+    return -1;
+}
+
+// resident workgroups of a kernel with `perCu` workgroups per CU on the current device (a multiple of kXcds)
+static int residentSlots(int perCu) {
+    static int cus[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (cus[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cus[dev] = n;
+    }
+    const int slots = cus[dev] * perCu;
+    return (slots + kXcds - 1) / kXcds * kXcds;
+}
+
 // LDS variant: tables in __shared__ memory, one source per workgroup of NT lanes, WGS workgroups per CU (each within
 // 160 KB / WGS of LDS). FC faces, VC vertices, QC ring records, TC targets; HF and HV face and vertex hash slots, which
 // share the ring's storage with the vertex ids during setup and with the force reduction afterwards; kEnd: the variant
@@ -1157,6 +1187,17 @@ __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P) {
     T.vcap = VC;
     T.qcap = QC;
     T.tcap = TC;
+    if (P.grab) {  // one workgroup per resident slot, sources taken dynamically (blockIdx.x % kXcds is the XCD)
+        const int x = blockIdx.x % kXcds;
+        for (;;) {
+            if (threadIdx.x == 0) sh.next = grabSource(P, x);
+            __syncthreads();
+            const int src = sh.next;
+            __syncthreads();  // every wave has read it before it is overwritten
+            if (src < 0) return;
+            solveSource(P, T, sh, src, nullptr, nullptr, 0);
+        }
+    }
     const int src = sourceOfBlock(P);
     if (src < 0) return;
     solveSource(P, T, sh, src, nullptr, nullptr, 0);
@@ -1317,10 +1358,11 @@ ScratchLayout makeScratchLayout(long faces, long targets) {
 
 template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, int WGS>
 static void launchLdsVariant(const GeoParams& P, hipStream_t stream) {
+    const int grid = P.grab ? std::min(residentSlots(WGS), (P.nSrc + kXcds - 1) / kXcds * kXcds) : gridForSources(P);
     if (P.pairEnd)
-        hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, true>), dim3(gridForSources(P)), dim3(NT), 0, stream, P);
+        hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, true>), dim3(grid), dim3(NT), 0, stream, P);
     else
-        hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, false>), dim3(gridForSources(P)), dim3(NT), 0, stream, P);
+        hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, false>), dim3(grid), dim3(NT), 0, stream, P);
 }
 
 hipError_t launchGeodesicLds(const GeoParams& P, int variant, hipStream_t stream) {
