@@ -48,9 +48,9 @@ struct GeoParams {
     const double* __restrict__ patchRadius;  // per source; nullptr: whole mesh (no submeshing)
     const int32_t* __restrict__ blockOrder;  // optional: source order (spatially sorted), split over the XCDs in chunks
     int blockChunk;                          // ceil(nSrc / kXcds) when blockOrder is set
-    // optional: kXcds counters, zero before the launch. With them the LDS kernel runs one workgroup per resident slot
-    // and each workgroup takes sources one at a time, from its own XCD's chunk of the order first, then from the
-    // other chunks, so that every XCD stays busy until the last sources (the chunks' costs differ by up to 25%).
+    // optional: kXcds counters, zero before the launch. With them each workgroup of the LDS kernel takes its source
+    // dynamically, from its own XCD's chunk of the order first, then from the other chunks, so that every XCD stays
+    // busy until the last sources (the chunks' costs differ by up to 25%).
     int* grab;
     const int32_t* __restrict__ patchList;   // optional: face sets from the patch kernel, patchStride per source
     const int32_t* __restrict__ patchCount;  // per source: faces listed, or -1 (the solver then searches itself)

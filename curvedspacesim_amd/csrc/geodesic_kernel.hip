@@ -676,7 +676,18 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int* cur
                            int nT) {
     const int lane = threadIdx.x;
     const DeviceMesh& M = P.mesh;
-    // A. the face set
+    // targets first, independent of the patch: weights normalised by their sum, and the global face id held in the
+    // best-distance word until phase C turns it into a face slot (so these loads overlap the face set's)
+    if (nT <= T.tcap)
+        for (int t = lane; t < nT; t += blockDim.x) {
+            const int pj = P.tidx[tb + t];
+            const double* b = P.pbary + 3 * pj;
+            const double sb = b[0] + b[1] + b[2];
+            for (int a = 0; a < 3; ++a) T.tq[3 * t + a] = b[a] / sb;
+            T.tbest[t] = (unsigned long long)(uint32_t)P.pface[pj];
+        }
+    // A. the face set (with a prepared list, also B. their corner vertices)
+    bool vertsDone = false;
     const int listed = (P.patchRadius && P.patchList) ? P.patchCount[src] : -1;
     if (P.patchRadius == nullptr) {
         for (int g = lane; g < M.nF; g += blockDim.x) {
@@ -688,8 +699,13 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int* cur
         if (listed > T.fcap) {
             if (lane == 0) atomicOr(&sh.overflow, OVF_FACES);
         } else {
-            for (int i = lane; i < listed; i += blockDim.x) faceInsertAt(T, lst[i], i);
+            for (int i = lane; i < listed; i += blockDim.x) {
+                const int g = lst[i];
+                faceInsertAt(T, g, i);
+                for (int k = 0; k < 3; ++k) vertInsert(T, sh, M.fv[3 * g + k]);
+            }
             if (lane == 0) sh.nFaces = listed;
+            vertsDone = true;
         }
     } else if (cur) {
         patchSearch(P, T, sh, cur, nxt, frontCap, src, sf, tb, nT);
@@ -698,12 +714,13 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int* cur
     }
     if (uniformOverflow(sh)) return false;
     const int nF = sh.nFaces;
-    // B. their corner vertices
-    for (int s = lane; s < nF; s += blockDim.x) {
-        const int g = T.fglob[s];
-        for (int k = 0; k < 3; ++k) vertInsert(T, sh, M.fv[3 * g + k]);
+    if (!vertsDone) {  // B. their corner vertices
+        for (int s = lane; s < nF; s += blockDim.x) {
+            const int g = T.fglob[s];
+            for (int k = 0; k < 3; ++k) vertInsert(T, sh, M.fv[3 * g + k]);
+        }
+        if (uniformOverflow(sh)) return false;
     }
-    if (uniformOverflow(sh)) return false;
     const int nV = sh.nVerts;
     // C. per face slot: corner and neighbour slots, corner counts per vertex; per target: face slot and weights
     for (int s = lane; s < nF; s += blockDim.x) {
@@ -719,11 +736,7 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int* cur
         T.thead[s] = -1;
     }
     for (int t = lane; t < nT; t += blockDim.x) {
-        const int pj = P.tidx[tb + t];
-        T.tslot[t] = faceLookup(T, P.pface[pj]);
-        const double* b = P.pbary + 3 * pj;
-        double sb = b[0] + b[1] + b[2];
-        for (int a = 0; a < 3; ++a) T.tq[3 * t + a] = b[a] / sb;
+        T.tslot[t] = faceLookup(T, (int)T.tbest[t]);
         T.tbest[t] = dbits(kInf);
     }
     if (lane == 0) {
@@ -1113,19 +1126,6 @@ __device__ int grabSource(const GeoParams& P, int x) {
     return -1;
 }
 
-// resident workgroups of a kernel with `perCu` workgroups per CU on the current device (a multiple of kXcds)
-static int residentSlots(int perCu) {
-    static int cus[64] = {};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-    if (cus[dev] == 0) {
-        int n = 0;
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
-        cus[dev] = n;
-    }
-    const int slots = cus[dev] * perCu;
-    return (slots + kXcds - 1) / kXcds * kXcds;
-}
 
 // LDS variant: tables in __shared__ memory, one source per workgroup of NT lanes, WGS workgroups per CU (each within
 // 160 KB / WGS of LDS). FC faces, VC vertices, QC ring records, TC targets; HF and HV face and vertex hash slots, which
@@ -1187,18 +1187,15 @@ __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P) {
     T.vcap = VC;
     T.qcap = QC;
     T.tcap = TC;
-    if (P.grab) {  // one workgroup per resident slot, sources taken dynamically (blockIdx.x % kXcds is the XCD)
-        const int x = blockIdx.x % kXcds;
-        for (;;) {
-            if (threadIdx.x == 0) sh.next = grabSource(P, x);
-            __syncthreads();
-            const int src = sh.next;
-            __syncthreads();  // every wave has read it before it is overwritten
-            if (src < 0) return;
-            solveSource(P, T, sh, src, nullptr, nullptr, 0);
-        }
+    int src;
+    if (P.grab) {  // the source is taken dynamically (blockIdx.x % kXcds is the XCD); surplus workgroups exit
+        if (threadIdx.x == 0) sh.next = grabSource(P, blockIdx.x % kXcds);
+        __syncthreads();
+        src = sh.next;
+        __syncthreads();
+    } else {
+        src = sourceOfBlock(P);
     }
-    const int src = sourceOfBlock(P);
     if (src < 0) return;
     solveSource(P, T, sh, src, nullptr, nullptr, 0);
 }
@@ -1358,7 +1355,9 @@ ScratchLayout makeScratchLayout(long faces, long targets) {
 
 template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, int WGS>
 static void launchLdsVariant(const GeoParams& P, hipStream_t stream) {
-    const int grid = P.grab ? std::min(residentSlots(WGS), (P.nSrc + kXcds - 1) / kXcds * kXcds) : gridForSources(P);
+    // dynamic distribution: twice the sources' workgroups, so that an XCD whose chunk costs less can take sources from
+    // the others (the surplus exits at once)
+    const int grid = P.grab ? 2 * kXcds * ((P.nSrc + kXcds - 1) / kXcds) : gridForSources(P);
     if (P.pairEnd)
         hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, true>), dim3(grid), dim3(NT), 0, stream, P);
     else
