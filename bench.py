@@ -7,6 +7,7 @@ evaluation, all resident in HBM. Prints one JSON line.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1|2|3] [--no-cpu-baseline]
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -24,13 +25,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, face, bary, vel, threads, budget_s=30.0):
+def cpu_baseline(w, face, bary, vel, threads, budget_s=30.0, accuracy=True):
     """The FP64 CPU restatement of the reference path (oracle, reference Release flags), timed on this host with
-    `threads` OpenMP threads, and on one thread (the reference's own execution model) over a smaller sample."""
+    `threads` OpenMP threads, and on one thread (the reference's own execution model) over a smaller sample. With
+    `accuracy`, the same restatement (IEEE build) also checks the GPU path from this state (accuracy_check)."""
     r = _cpu_time(w, face, bary, vel, threads, budget_s)
     if threads > 1:
         one = _cpu_time(w, face, bary, vel, 1, budget_s / 3)
         r["single_thread"] = {"value": one["value"], "unit": one["unit"], "cores": 1, "sample": one["sample"]}
+    if accuracy:
+        log("accuracy of the GPU path against the restatement")
+        r["accuracy_of_gpu_path"] = accuracy_check(w, face, bary, vel, threads)
     return r
 
 
@@ -148,6 +153,54 @@ def screened_leg(ctx, w, sysm, steps, plain_steps_s):
                      "note": "MLP forward flops over the whole screening launch group (features, forward, compaction)"}}
 
 
+def accuracy_check(w, face, bary, vel, threads):
+    """Errors against the oracle from the benchmark's current state, in a context of its own: candidate lists, pair
+    distances (relative over pairs within the cutoff with d > 1e-3 rc, absolute below) and forces of one computation,
+    then positions and velocities after one step taken by both from that state (fresh thermostat chains)."""
+    import curvedspacesim_amd as csa
+    from curvedspacesim_amd import workloads
+    from oracle.pyoracle import OracleMesh, OracleSystem
+
+    t0 = time.perf_counter()
+    actx = csa.Context(0)
+    ws = dataclasses.replace(w, face=face, bary=bary, vel=vel)
+    sysm = workloads.make_system(actx, ws)
+    sysm.record_pairs(True)
+    F, E = sysm.compute_forces()
+    off, idx = sysm.pairs()
+    d = sysm.pair_distances()
+    om = OracleMesh(w.X, w.T)
+    osys = OracleSystem(om, face, bary, vel, w.potential, w.range, w.cell_list, w.submeshing, w.maximum_distance,
+                        threads=threads)
+    r = osys.compute_forces()
+    same_lists = bool(np.array_equal(off, osys.last["off"]) and np.array_equal(idx, osys.last["idx"]))
+    do = r["dist"]
+    within = do < w.range
+    big = within & (do > 1e-3 * w.range)
+    rel = float(np.max(np.abs(d[big] - do[big]) / do[big])) if np.any(big) else 0.0
+    small = within & ~big
+    absd = float(np.max(np.abs(d[small] - do[small]))) if np.any(small) else 0.0
+    fl = r["flags"]
+    tie = np.array([np.any(fl[off[i]:off[i + 1]] & 2) for i in range(len(face))])  # cut-locus pairs (tangent ambiguous)
+    scale = np.abs(r["forces"]).max()
+    dF = np.abs(F - r["forces"]).max(1) / scale
+    sysm.record_pairs(False)
+    workloads.step(sysm, ws, 1)
+    if w.integrator == "nvt":
+        osys.nvt_step(w.dt, w.temperature, w.tau, w.chain)
+    else:
+        osys.nve_step(w.dt)
+    f1, b1, v1, _ = sysm.get()
+    dx = float(np.max(np.abs(om.positions(f1, b1) - om.positions(osys.face, osys.bary))))
+    dv = float(np.max(np.abs(v1 - osys.vel)) / np.abs(osys.vel).max())
+    actx.close()
+    return {"pairs_compared": int(np.sum(within)), "candidate_lists_identical": same_lists,
+            "max_rel_distance_error": rel, "max_abs_distance_error_small_d": absd,
+            "max_force_error_rel": float(dF.max()), "max_force_error_rel_excluding_cut_locus": float(dF[~tie].max()),
+            "sources_with_cut_locus_pairs": int(tie.sum()), "one_step_max_position_error": dx,
+            "one_step_max_velocity_error_rel": dv, "oracle_threads": threads, "seconds": time.perf_counter() - t0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +211,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-screen", action="store_true", help="skip the learned-screen leg")
     ap.add_argument("--no-graph", action="store_true", help="launch every kernel directly instead of replaying a captured step")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the CPU baseline's check of the GPU path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}): this build measures one GPU's share of the "
@@ -245,7 +299,7 @@ def main():
         f, b, v, _ = sysm.get()
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline with {threads} threads")
-        out["cpu_baseline"] = cpu_baseline(w, f, b, v, threads)
+        out["cpu_baseline"] = cpu_baseline(w, f, b, v, threads, accuracy=not args.no_accuracy)
         out["speedup_vs_cpu"] = steps_s / out["cpu_baseline"]["value"]
     else:
         out["cpu_baseline"] = None
