@@ -295,7 +295,10 @@ def main():
     }
     if not args.no_screen and args.config == 3:
         out["screened"] = screened_leg(ctx, w, sysm, args.steps, steps_s)
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and w.meta.get("groups") is not None:
+        log("no CPU baseline for particle groups: the restatement runs one system at a time")
+        out["cpu_baseline"] = None
+    elif not args.no_cpu_baseline:
         f, b, v, _ = sysm.get()
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline with {threads} threads")

@@ -82,6 +82,7 @@ def csg():
     L.csg_system_set_lds_variant.argtypes = [VP, C.c_int32]
     L.csg_system_use_graph.argtypes = [VP, C.c_int32]
     L.csg_system_record_pairs.argtypes = [VP, C.c_int32]
+    L.csg_system_set_groups.argtypes = [VP, C.c_int32, VP, VP, VP, VP]
     L.csg_system_pair_distances.argtypes = [VP, _f64p, C.c_int64]
     L.csg_system_screen_features.argtypes = [VP, _f32p, C.c_int64]
     L.csg_system_set_screen.argtypes = [VP, VP, C.c_float]
@@ -372,6 +373,21 @@ class System:
 
     def set_lds_variant(self, variant):
         self.ctx._check(self.L.csg_system_set_lds_variant(self.ctx.h, int(variant)), "csg_system_set_lds_variant")
+
+    def set_groups(self, group_of, potentials, ranges, maximum_distances):
+        """Independent particle groups (see csg_system_set_groups): group index per particle, and per group a potential
+        tuple, an interaction range and a maximum distance. group_of=None clears."""
+        if group_of is None:
+            self.ctx._check(self.L.csg_system_set_groups(self.ctx.h, 0, None, None, None, None), "csg_system_set_groups")
+            return
+        g = np.ascontiguousarray(group_of, np.int32)
+        n = len(potentials)
+        pots = (csg_potential * n)(*[csg_potential(int(p[0]), *[float(x) for x in (list(p[1:]) + [0, 0, 0])[:3]])
+                                     for p in potentials])
+        r = np.ascontiguousarray(ranges, np.float64)
+        m = np.ascontiguousarray(maximum_distances, np.float64)
+        self.ctx._check(self.L.csg_system_set_groups(self.ctx.h, n, _ptr(g), pots, _ptr(r), _ptr(m)),
+                        "csg_system_set_groups")
 
     def record_pairs(self, enable=True):
         self.ctx._check(self.L.csg_system_record_pairs(self.ctx.h, int(bool(enable))), "csg_system_record_pairs")

@@ -40,6 +40,12 @@ struct SystemState {
     DevBuf scratch;
     // learned screen: features and logits of the candidate pairs, kept-pair lists; recorded pair distances
     DevBuf sfeat, slogit, scnt, soff, scand, sparams, pdist;
+    // independent particle groups: group of each particle; per group potential, squared range, patch-radius cap,
+    // truncation radius and sentinel distance
+    int nGroups = 0;
+    DevBuf grp, gpot, gr2, gmaxd, grmax, gsent;
+    std::vector<Potential> hgpot;
+    std::vector<double> hgrange, hgmaxd;
     bool screenOn = false, recordPairs = false;
     int ldsVariant = 0;  // workgroup size of the LDS geodesic kernel, chosen by csg_system_autotune
     // one MD step captured as a HIP graph (replayed per step), and what it was captured for
@@ -97,7 +103,8 @@ void destroySystem(SystemState* s) {
     DevBuf* all[] = {&s->face, &s->bary, &s->vel, &s->force, &s->mass, &s->energy, &s->xyz, &s->sxyz, &s->key, &s->cellCount, &s->cellStart,
                      &s->cellFill, &s->order, &s->cnt, &s->off, &s->toff, &s->cand, &s->maxd, &s->rad, &s->disp, &s->tstatus,
                      &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->spill, &s->grab, &s->scratch,
-                     &s->sfeat, &s->slogit, &s->scnt, &s->soff, &s->scand, &s->sparams, &s->pdist};
+                     &s->sfeat, &s->slogit, &s->scnt, &s->soff, &s->scand, &s->sparams, &s->pdist,
+                     &s->grp, &s->gpot, &s->gr2, &s->gmaxd, &s->grmax, &s->gsent};
     for (DevBuf* b : all) b->release();
     for (auto& m : s->marks) {
         hipEventDestroy(m.a);
@@ -208,13 +215,21 @@ __global__ void kSortCells(int nCells, const int* cellStart, int* order) {
 // cell), one wavefront per particle: the lanes test consecutive members of each stencil cell, and a ballot prefix places
 // the hits in that order. Wavefront t takes the t-th particle in cell order and reads positions from their cell-ordered
 // copy sxyz, contiguous within a cell.
+// With particle groups (grp non-null) only particles of the same group are candidates, within that group's range
+// (gr2: squared) and with its patch-radius cap (gmaxd).
 template <bool kFill>
 __global__ void kCandidatesWave(Grid g, int N, const double* sxyz, const int* key, const int* cellStart, const int* order, double r2,
-                                int* cnt, const int* off, long cap, int* cand, double* maxd, double* rad, double maxDist, int* counters) {
+                                int* cnt, const int* off, long cap, int* cand, double* maxd, double* rad, double maxDist, int* counters,
+                                const int* grp, const double* gr2, const double* gmaxd) {
     const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (t >= N) return;
     const int i = order[t];
+    const int gi = grp ? grp[i] : 0;
+    if (grp) {
+        r2 = gr2[gi];
+        maxDist = gmaxd[gi];
+    }
     const double p[3] = {sxyz[3 * t], sxyz[3 * t + 1], sxyz[3 * t + 2]};
     const int c = key[i];
     const int cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / (g.n[0] * g.n[1]);
@@ -234,7 +249,7 @@ __global__ void kCandidatesWave(Grid g, int N, const double* sxyz, const int* ke
                     if (q < ce && q != t) {
                         const double dx = p[0] - sxyz[3 * q], dy = p[1] - sxyz[3 * q + 1], dz = p[2] - sxyz[3 * q + 2];
                         d2 = dx * dx + dy * dy + dz * dz;
-                        hit = d2 < r2;
+                        hit = d2 < r2 && (grp == nullptr || grp[order[q]] == gi);
                     }
                     const unsigned long long mask = __ballot(hit);
                     if (hit) {
@@ -461,13 +476,16 @@ static csg_status sysNeighbors(csg_ctx* c, SystemState* S) {
     hipLaunchKernelGGL(kGatherPositions, dim3(blocksFor(N)), dim3(kBlock), 0, st, N, S->order.as<int>(), S->xyz.as<double>(),
                        S->sxyz.as<double>());
     double r2 = S->range * S->range;
+    const int* grp = S->nGroups ? S->grp.as<int>() : nullptr;
+    const double* gr2 = S->nGroups ? S->gr2.as<double>() : nullptr;
+    const double* gmaxd = S->nGroups ? S->gmaxd.as<double>() : nullptr;
     hipLaunchKernelGGL((kCandidatesWave<false>), dim3(blocksFor(64L * N)), dim3(kBlock), 0, st, g, N, S->sxyz.as<double>(), S->key.as<int>(),
                        S->cellStart.as<int>(), S->order.as<int>(), r2, S->cnt.as<int>(), (const int*)nullptr, S->candCap, (int*)nullptr,
-                       (double*)nullptr, (double*)nullptr, c->maximumDistance, S->counters.as<int>());
+                       (double*)nullptr, (double*)nullptr, c->maximumDistance, S->counters.as<int>(), grp, gr2, gmaxd);
     hipLaunchKernelGGL(kScanExclusive, dim3(1), dim3(1024), 0, st, S->cnt.as<int>(), S->off.as<int>(), N);
     hipLaunchKernelGGL((kCandidatesWave<true>), dim3(blocksFor(64L * N)), dim3(kBlock), 0, st, g, N, S->sxyz.as<double>(), S->key.as<int>(),
                        S->cellStart.as<int>(), S->order.as<int>(), r2, S->cnt.as<int>(), S->off.as<int>(), S->candCap, S->cand.as<int>(),
-                       S->maxd.as<double>(), S->rad.as<double>(), c->maximumDistance, S->counters.as<int>());
+                       S->maxd.as<double>(), S->rad.as<double>(), c->maximumDistance, S->counters.as<int>(), grp, gr2, gmaxd);
     hipLaunchKernelGGL(kClampOffsets, dim3(blocksFor(N + 1)), dim3(kBlock), 0, st, N, S->off.as<int>(), S->candCap, S->toff.as<int>(),
                        S->counters.as<int>());
     HIPCHECK(c, hipGetLastError());
@@ -532,6 +550,12 @@ static csg_status sysForces(csg_ctx* c, SystemState* S) {
     G.rMax = exactTruncation(S->pot, S->rMax, G.useSentinel != 0, G.sentinelDist);
     G.patchStats = S->pstat.as<int>();
     G.pairDist = S->recordPairs ? S->pdist.as<double>() : nullptr;
+    if (S->nGroups) {  // per-group potential, truncation radius and sentinel
+        G.srcGroup = S->grp.as<int32_t>();
+        G.groupPot = S->gpot.as<Potential>();
+        G.groupRMax = S->grmax.as<double>();
+        G.groupSentinel = S->gsent.as<double>();
+    }
     if (S->cellList) {  // sources in cell order, one contiguous chunk per XCD
         G.blockOrder = S->order.as<int32_t>();
         G.blockChunk = (S->N + kXcds - 1) / kXcds;
@@ -737,6 +761,7 @@ csg_status csg_system_configure(csg_ctx* c, const csg_potential* pot, double ran
     dropGraph(S);  // buffers may move
     S->pot = pot ? Potential{pot->kind, pot->p0, pot->p1, pot->p2} : Potential{POT_NONE, 0, 0, 0};
     S->range = range;
+    S->nGroups = 0;  // particle groups are set again after a new configuration
     S->cellList = cellList ? 1 : 0;
     S->rMax = rMax;
     const int N = S->N;
@@ -1050,11 +1075,65 @@ csg_status csg_system_screen_features(csg_ctx* c, float* features, int64_t cap) 
     return CSG_OK;
 }
 
+// per-group truncation radius and sentinel from the group's potential, range and maximum distance (as exactTruncation
+// does for a single system whose truncation radius is its range)
+static csg_status uploadGroupRadii(csg_ctx* c, SystemState* S) {
+    std::vector<double> rmax(S->nGroups), sent(S->nGroups);
+    for (int g = 0; g < S->nGroups; ++g) {
+        sent[g] = 2.0 * S->hgmaxd[g];
+        rmax[g] = exactTruncation(S->hgpot[g], S->hgrange[g], c->submeshing != 0, sent[g]);
+    }
+    HIPCHECK(c, hipMemcpy(S->grmax.p, rmax.data(), 8 * (size_t)S->nGroups, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(S->gsent.p, sent.data(), 8 * (size_t)S->nGroups, hipMemcpyHostToDevice));
+    return CSG_OK;
+}
+
+csg_status csg_system_set_groups(csg_ctx* c, int32_t nGroups, const int32_t* groupOf, const csg_potential* pots, const double* ranges,
+                                 const double* maximumDistances) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S || !S->configured) return fail(c, CSG_ERR_STATE, "csg_system_init and csg_system_configure first");
+    dropGraph(S);
+    HIPCHECK(c, hipSetDevice(c->device));
+    if (nGroups <= 0) {
+        S->nGroups = 0;
+        return CSG_OK;
+    }
+    if (!groupOf || !pots || !ranges || !maximumDistances) return fail(c, CSG_ERR_ARGUMENT, "null group arrays");
+    if (!S->cellList) return fail(c, CSG_ERR_STATE, "particle groups need cell-list candidates");
+    if (S->screenOn) return fail(c, CSG_ERR_STATE, "the learned screen is on");
+    for (int i = 0; i < S->N; ++i)
+        if (groupOf[i] < 0 || groupOf[i] >= nGroups) return fail(c, CSG_ERR_ARGUMENT, "group index out of range");
+    for (int g = 0; g < nGroups; ++g)
+        if (!(ranges[g] > 0) || ranges[g] > S->range) return fail(c, CSG_ERR_ARGUMENT, "a group range is not in (0, configured range]");
+    S->hgpot.resize(nGroups);
+    S->hgrange.assign(ranges, ranges + nGroups);
+    S->hgmaxd.assign(maximumDistances, maximumDistances + nGroups);
+    std::vector<double> r2(nGroups);
+    for (int g = 0; g < nGroups; ++g) {
+        S->hgpot[g] = Potential{pots[g].kind, pots[g].p0, pots[g].p1, pots[g].p2};
+        r2[g] = ranges[g] * ranges[g];
+    }
+    HIPCHECK(c, S->grp.ensure(4 * (size_t)S->N));
+    HIPCHECK(c, S->gpot.ensure(sizeof(Potential) * (size_t)nGroups));
+    HIPCHECK(c, S->gr2.ensure(8 * (size_t)nGroups));
+    HIPCHECK(c, S->gmaxd.ensure(8 * (size_t)nGroups));
+    HIPCHECK(c, S->grmax.ensure(8 * (size_t)nGroups));
+    HIPCHECK(c, S->gsent.ensure(8 * (size_t)nGroups));
+    HIPCHECK(c, hipMemcpy(S->grp.p, groupOf, 4 * (size_t)S->N, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(S->gpot.p, S->hgpot.data(), sizeof(Potential) * (size_t)nGroups, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(S->gr2.p, r2.data(), 8 * (size_t)nGroups, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(S->gmaxd.p, maximumDistances, 8 * (size_t)nGroups, hipMemcpyHostToDevice));
+    S->nGroups = nGroups;
+    return uploadGroupRadii(c, S);
+}
+
 csg_status csg_system_set_screen(csg_ctx* c, const float* params, float threshold) {
     SystemState* S = sysOf(c);
     if (!c) return CSG_ERR_ARGUMENT;
     if (!S || !S->configured) return fail(c, CSG_ERR_STATE, "csg_system_init and csg_system_configure first");
     if (params && (!S->cellList || !c->submeshing)) return fail(c, CSG_ERR_STATE, "the screen needs cell-list candidates and submeshing");
+    if (params && S->nGroups) return fail(c, CSG_ERR_STATE, "the screen is unavailable while particle groups are set");
     dropGraph(S);
     HIPCHECK(c, hipSetDevice(c->device));
     if (!params) {

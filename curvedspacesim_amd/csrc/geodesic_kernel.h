@@ -71,6 +71,12 @@ struct GeoParams {
     Potential pot;
     int useSentinel;    // unreachable pairs get (sentinelDist, (0,0,1)) as the reference does
     double sentinelDist;
+    // optional particle groups: group of each particle; per group the potential, truncation radius and sentinel
+    // distance that replace pot, rMax and sentinelDist for the group's sources
+    const int32_t* srcGroup;
+    const Potential* groupPot;
+    const double* groupRMax;
+    const double* groupSentinel;
 };
 
 struct ScratchLayout {

@@ -768,10 +768,10 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int* cur
 }
 
 template <class TT>
-__device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf, int tb, SpillRecord* spill) {
+__device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf, int tb, SpillRecord* spill, double rMax) {
     const int lane = threadIdx.x;
     const DeviceMesh& M = P.mesh;
-    Ctx<TT> C{P, T, sh, 0, P.rMax, tb, spill};
+    Ctx<TT> C{P, T, sh, 0, rMax, tb, spill};
     const int ss = sh.srcSlot;
     // corners of the source face are visible directly
     if (lane < 3) {
@@ -941,6 +941,11 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
     const int pi = P.srcIdx ? P.srcIdx[src] : src;
     const int sf = P.pface[pi];
     const int tb = P.toff[src], nT = P.toff[src + 1] - tb;
+    // the source's potential, truncation radius and sentinel (its group's, with particle groups)
+    const int grp = P.srcGroup ? P.srcGroup[pi] : -1;
+    const Potential pot = grp >= 0 ? P.groupPot[grp] : P.pot;
+    const double rMax = grp >= 0 ? P.groupRMax[grp] : P.rMax;
+    const double sentinelDist = grp >= 0 ? P.groupSentinel[grp] : P.sentinelDist;
     for (int i = lane; i <= T.hmask; i += blockDim.x) T.hkey[i] = -1;
     for (int i = lane; i <= T.vhmask; i += blockDim.x) T.vhkey[i] = -1;
     for (int i = lane; i < T.vcap; i += blockDim.x) T.vfo[i] = 0;
@@ -958,7 +963,7 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
     const long long c0 = wall_clock64();
     if (ok) ok = buildPatch(P, T, sh, cur, nxt, frontCap, src, sf, tb, nT);
     const long long c1 = wall_clock64();
-    if (ok) propagateAll(P, T, sh, sf, tb, spill);
+    if (ok) propagateAll(P, T, sh, sf, tb, spill, rMax);
     __syncthreads();
     const long long c2 = wall_clock64();
     // ---- finalize: paths through the target face's corners, direct visibility, outputs, force
@@ -1022,11 +1027,11 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
             et[0] = et[1] = et[2] = 0;
             if (P.useSentinel) {
                 flags |= PAIR_SENTINEL;
-                d = P.sentinelDist;
+                d = sentinelDist;
                 s3[2] = 1;
                 et[2] = 1;
             }
-        } else if (best > P.rMax) {
+        } else if (best > rMax) {
             flags |= PAIR_BEYOND;
         }
         if (P.pairDist) P.pairDist[out] = d;
@@ -1041,9 +1046,9 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
             P.pairEnd[3 * out + 2] = (float)et[2];
         }
         if (P.pairFlags) P.pairFlags[out] = flags;
-        if (P.pot.kind != POT_NONE && (!(flags & PAIR_UNREACHED) || (flags & PAIR_SENTINEL))) {
+        if (pot.kind != POT_NONE && (!(flags & PAIR_UNREACHED) || (flags & PAIR_SENTINEL))) {
             double e;
-            double sc = potentialScale(P.pot, d, &e);
+            double sc = potentialScale(pot, d, &e);
             F0 += sc * s3[0];
             F1 += sc * s3[1];
             F2 += sc * s3[2];

@@ -100,7 +100,59 @@ def config(k, seed=13377, n=None):
         return Workload("bumpy sphere L6 (81,920 faces), N=20,000, truncated Gaussian s=0.05 rc=0.15, NH-NVT T=0.5", X, T,
                         f, b, tangent_velocities(X, T, f, 0.5, rng), (nat.TRUNC_GAUSSIAN, 1.0, s, rc), rc, True, True,
                         rc, rc, "nvt", 0.005, 0.5, 1.0, 2)
+    if k == 5:
+        return replicas(seed, count=32)
     raise ValueError(f"unknown configuration {k}")
+
+
+REPLICA_MESHES = ("icosphere L5", "jittered torus", "bumpy sphere L6")
+
+
+def replica_mesh(kind, seed):
+    if kind == 0:
+        return nat.icosphere(5)
+    if kind == 1:
+        return nat.jittered_torus(3.0, 1.0, 200, 100, 0.02, seed)
+    return nat.bumpy_sphere(6, 20, 0.15, 0.3, seed)
+
+
+def replicas(seed=13377, count=32, n_min=256, n_max=8192, spacing=12.0, kinds=None):
+    """Many uneven independent systems stepped together: `count` replicas, each on a seeded mesh from {icosphere L5,
+    jittered torus (40,000 faces), bumpy sphere L6}, with N drawn log-uniformly from [n_min, n_max], area-uniform
+    placement, harmonic repulsion with sigma ~ U(0.05, 0.3) (patch radius and range sigma), velocities at T = 0.1, VV-NVE.
+    The replicas' meshes are uploaded as one union, replica r translated by r * spacing along x so that no two replicas'
+    particles are Euclidean neighbours; each replica is a particle group (csg_system_set_groups). meta["replicas"]
+    keeps every replica's own mesh and index ranges."""
+    rng = np.random.default_rng(seed)
+    Xs, Ts, fs, bs, vs, groups, pots, ranges, reps = [], [], [], [], [], [], [], [], []
+    v0 = f0 = p0 = 0
+    for r in range(count):
+        kind = int(rng.integers(0, 3)) if kinds is None else int(kinds[r % len(kinds)])
+        X, T = replica_mesh(kind, seed + 1 + r)
+        n = int(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        sigma = float(rng.uniform(0.05, 0.3))
+        f, b = area_uniform(X, T, n, rng)
+        v = tangent_velocities(X, T, f, 0.1, rng)
+        Xs.append(X + np.array([r * spacing, 0.0, 0.0]))
+        Ts.append(T + v0)
+        fs.append(f + f0)
+        bs.append(b)
+        vs.append(v)
+        groups.append(np.full(n, r, np.int32))
+        pots.append((nat.HARMONIC, 1.0 / sigma ** 2, sigma))
+        ranges.append(sigma)
+        reps.append(dict(mesh=REPLICA_MESHES[kind], X=X, T=T, n=n, sigma=sigma, vert_offset=v0, face_offset=f0,
+                         particle_offset=p0))
+        v0 += len(X)
+        f0 += len(T)
+        p0 += n
+    smax = max(ranges)
+    return Workload(f"{count} replicas on icosphere L5 / jittered torus / bumpy sphere L6, N={p0} in total "
+                    f"(log-uniform {n_min}-{n_max} each), harmonic sigma ~ U(0.05, 0.3), VV-NVE",
+                    np.concatenate(Xs), np.concatenate(Ts).astype(np.int32), np.concatenate(fs).astype(np.int32),
+                    np.concatenate(bs), np.concatenate(vs), (nat.HARMONIC, 1.0 / smax ** 2, smax), smax, True, True, smax,
+                    smax, "nve", 0.005, 0.1,
+                    meta=dict(groups=np.concatenate(groups), pots=pots, ranges=ranges, maxd=list(ranges), replicas=reps))
 
 
 def make_system(ctx, w, max_patch_faces=0):
@@ -108,6 +160,8 @@ def make_system(ctx, w, max_patch_faces=0):
     ctx.set_submeshing(w.submeshing, w.maximum_distance)
     sysm = nat.System(ctx, w.face, w.bary, w.vel)
     sysm.configure(w.potential, w.range, w.cell_list, w.rmax, max_patch_faces)
+    if w.meta.get("groups") is not None:
+        sysm.set_groups(w.meta["groups"], w.meta["pots"], w.meta["ranges"], w.meta["maxd"])
     return sysm
 
 

@@ -160,6 +160,15 @@ csg_status csg_system_use_graph(csg_ctx* ctx, int32_t enable);
 csg_status csg_system_set_lds_variant(csg_ctx* ctx, int32_t variant);
 /* candidate lists of the last force computation (offsets N+1, indices up to cap), before any screening */
 csg_status csg_system_pairs(csg_ctx* ctx, int32_t* offsets, int32_t* idx, int64_t cap);
+/* independent particle groups in one system (no reference counterpart: many independent simulations stepped by the
+ * same launches, each as the reference would run it alone). groupOf[i] in [0, nGroups) per particle; per group its
+ * potential, interaction range (the cell list's candidate cutoff, at most the configured range, which sets the cell
+ * size) and maximum distance (patch-radius cap and sentinel, as csg_set_submeshing's for a single system). Particles of
+ * different groups are never candidates of each other, so groups on disjoint meshes of one uploaded union do not
+ * interact. A group's range is also its truncation radius (raised as rMax is for a single system). Requires the cell
+ * list; the learned screen is unavailable while groups are set. nGroups = 0 clears, and so does csg_system_configure. */
+csg_status csg_system_set_groups(csg_ctx* ctx, int32_t nGroups, const int32_t* groupOf, const csg_potential* pots, const double* ranges,
+                                 const double* maximumDistances);
 /* keep the geodesic distance of every candidate pair of later force computations (no reference counterpart; the
  * training data of the learned screen); csg_system_pair_distances copies those of the last one (screen off) */
 csg_status csg_system_record_pairs(csg_ctx* ctx, int32_t enable);
