@@ -36,7 +36,7 @@ struct SystemState {
     DevBuf face, bary, vel, force, mass, energy;
     DevBuf xyz, sxyz, key, cellCount, cellStart, cellFill, order;
     DevBuf cnt, off, toff, cand, maxd, rad;
-    DevBuf disp, tstatus, gstatus, partial, counters, ke, chain, pstat, plist, pcount, spill, grab;
+    DevBuf disp, tstatus, gstatus, partial, counters, ke, chain, pstat, plist, pcount, spill, grab, redo, tspill;
     DevBuf scratch;
     // learned screen: features and logits of the candidate pairs, kept-pair lists; recorded pair distances
     DevBuf sfeat, slogit, scnt, soff, scand, sparams, pdist;
@@ -102,7 +102,7 @@ void destroySystem(SystemState* s) {
     if (s->graph) hipGraphExecDestroy(s->graph);
     DevBuf* all[] = {&s->face, &s->bary, &s->vel, &s->force, &s->mass, &s->energy, &s->xyz, &s->sxyz, &s->key, &s->cellCount, &s->cellStart,
                      &s->cellFill, &s->order, &s->cnt, &s->off, &s->toff, &s->cand, &s->maxd, &s->rad, &s->disp, &s->tstatus,
-                     &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->spill, &s->grab, &s->scratch,
+                     &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->spill, &s->grab, &s->redo, &s->tspill, &s->scratch,
                      &s->sfeat, &s->slogit, &s->scnt, &s->soff, &s->scand, &s->sparams, &s->pdist,
                      &s->grp, &s->gpot, &s->gr2, &s->gmaxd, &s->grmax, &s->gsent};
     for (DevBuf* b : all) b->release();
@@ -584,6 +584,12 @@ static csg_status sysForces(csg_ctx* c, SystemState* S) {
         }
         hipLaunchKernelGGL(kCountReasons, dim3(blocksFor(S->N)), dim3(kBlock), 0, c->stream, S->N, S->gstatus.as<int>(), S->counters.as<int>());
         MarkScope mark(S, c->stream, MARK_GEODESIC_HBM);
+        // the sources the LDS variant could not hold: two tiers of larger LDS tables, then HBM tables for what remains
+        for (int tier = 1; tier <= 2; ++tier) {
+            hipLaunchKernelGGL(kFillInt, dim3(1), dim3(kBlock), 0, c->stream, S->redo.as<int>(), 0, 1L);
+            HIPCHECK(c, launchGeodesicLdsLarge(G, tier, S->redo.as<int32_t>() + 1, S->redo.as<int32_t>(), S->tspill.as<SpillRecord>(),
+                                               c->stream));
+        }
         HIPCHECK(c, launchGeodesicScratchRedo(G, S->layout, S->scratchBlocks, c->stream));
     } else {
         MarkScope mark(S, c->stream, MARK_GEODESIC_HBM);
@@ -744,6 +750,8 @@ csg_status csg_system_init(csg_ctx* c, int32_t N, const int32_t* face, const dou
     HIPCHECK(c, S->pcount.ensure(4 * (size_t)N));
     HIPCHECK(c, S->spill.ensure(sizeof(SpillRecord) * (size_t)kSpillCap * N));
     HIPCHECK(c, S->grab.ensure(sizeof(int) * kXcds));
+    HIPCHECK(c, S->redo.ensure(sizeof(int) * ((size_t)N + 1)));
+    HIPCHECK(c, S->tspill.ensure(kTierSpillBytes));
     HIPCHECK(c, S->partial.ensure(8 * (size_t)blocksFor(N)));
     HIPCHECK(c, S->counters.ensure(4 * CNT_N));
     HIPCHECK(c, S->ke.ensure(16));

@@ -57,6 +57,7 @@ struct GeoParams {
     int patchStride;
     SpillRecord* spill;                  // optional: spillCap records per source for work items beyond the LDS ring
     int spillCap;
+    int spillPerBlock;                   // 1: spillCap records per workgroup instead (persistent list kernels)
     double rMax;                         // windows beyond this lower-bound distance are not propagated
     // outputs, all optional
     double* pairDist;
@@ -97,6 +98,12 @@ hipError_t launchPatch(const GeoParams& P, int32_t* list, int32_t* count, int st
 // and 128 targets
 constexpr int kLdsVariants = CSG_LDS_VARIANTS;
 hipError_t launchGeodesicLds(const GeoParams& P, int variant, hipStream_t stream);
+// the sources flagged for a table capacity (P.status), solved again with larger LDS tables, persistent over their
+// list: tier 1 up to 1,024 faces (two workgroups per CU), tier 2 up to 1,536 faces (one per CU, searching the patch
+// itself when the patch kernel could not list it). list: nSrc ints,
+// count: one int, zero before the call. spill: kTierSpillBytes, shared by the workgroups (P.spill is not used)
+constexpr size_t kTierSpillBytes = (size_t)512 * 8192 * sizeof(SpillRecord);
+hipError_t launchGeodesicLdsLarge(const GeoParams& P, int tier, int32_t* list, int32_t* count, SpillRecord* spill, hipStream_t stream);
 // fills mesh.fchild (out, 18 per face) from mesh.fgeo
 hipError_t launchChildFrames(const DeviceMesh& mesh, double* out, hipStream_t stream);
 hipError_t launchGeodesicScratch(const GeoParams& P, const ScratchLayout& L, const int32_t* list, int nList, int blocks,

@@ -956,7 +956,7 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
         sh.nCur = sh.nNext = sh.goals = 0;
         initSourceFrame(P, sh, pi, sf);
     }
-    SpillRecord* spill = P.spill ? P.spill + (size_t)src * P.spillCap : nullptr;
+    SpillRecord* spill = P.spill ? P.spill + (size_t)(P.spillPerBlock ? blockIdx.x : src) * P.spillCap : nullptr;
     __syncthreads();
     bool ok = !sh.overflow;
     __syncthreads();
@@ -1136,8 +1136,8 @@ __device__ int grabSource(const GeoParams& P, int x) {
 // 160 KB / WGS of LDS). FC faces, VC vertices, QC ring records, TC targets; HF and HV face and vertex hash slots, which
 // share the ring's storage with the vertex ids during setup and with the force reduction afterwards; kEnd: the variant
 // that also reports end tangents.
-template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, int WGS, bool kEnd>
-__global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P) {
+template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, int WGS, bool kEnd, bool kList, int FR = 0>
+__global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P, const int32_t* list, const int32_t* count) {
     constexpr int kRingBytes = QC * (5 * 8 + 4 + 8);
     static_assert(HF * 6 + HV * 6 + VC * 4 + FC * 4 <= kRingBytes, "setup hashes, vertex ids and list heads live in the ring's storage");
     static_assert(QC >= 4 * kMaxWaves, "the epilogue's reduction uses the ring's storage");
@@ -1156,6 +1156,7 @@ __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P) {
     __shared__ unsigned long long tbest[TC];
     __shared__ float2 tst[TC];
     __shared__ float4 tet[kEnd ? TC : 1];
+    __shared__ int32_t front[FR > 0 ? 2 * FR : 1];  // breadth-first frontiers for sources without a prepared face list
     __shared__ Shared sh;
     Tables<int16_t> T;
     T.qb0 = (double*)ring;
@@ -1192,6 +1193,12 @@ __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P) {
     T.vcap = VC;
     T.qcap = QC;
     T.tcap = TC;
+    if constexpr (kList) {  // persistent over a device list of sources (the larger tables' pass)
+        const int n = *count;
+        for (int i = blockIdx.x; i < n; i += gridDim.x)
+            solveSource(P, T, sh, list[i], FR > 0 ? front : nullptr, FR > 0 ? front + FR : nullptr, FR);
+        return;
+    }
     int src;
     if (P.grab) {  // the source is taken dynamically (blockIdx.x % kXcds is the XCD); surplus workgroups exit
         if (threadIdx.x == 0) sh.next = grabSource(P, blockIdx.x % kXcds);
@@ -1360,13 +1367,50 @@ ScratchLayout makeScratchLayout(long faces, long targets) {
 
 template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, int WGS>
 static void launchLdsVariant(const GeoParams& P, hipStream_t stream) {
+    const int32_t* none = nullptr;
     // dynamic distribution: twice the sources' workgroups, so that an XCD whose chunk costs less can take sources from
     // the others (the surplus exits at once)
     const int grid = P.grab ? 2 * kXcds * ((P.nSrc + kXcds - 1) / kXcds) : gridForSources(P);
     if (P.pairEnd)
-        hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, true>), dim3(grid), dim3(NT), 0, stream, P);
+        hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, true, false>), dim3(grid), dim3(NT), 0, stream, P, none, none);
     else
-        hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, false>), dim3(grid), dim3(NT), 0, stream, P);
+        hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, false, false>), dim3(grid), dim3(NT), 0, stream, P, none, none);
+}
+
+// sources whose first solve overflowed a capacity of the LDS tables (not the step limit) into list[0 .. *count)
+__global__ void listOverflowedKernel(int n, const int* status, int32_t* list, int32_t* count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (status[i] & (OVF_FACES | OVF_VERTS | OVF_QUEUE | OVF_TARGETS)) list[atomicAdd(count, 1)] = i;
+}
+
+template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, int WGS, int FR>
+static void launchLdsList(const GeoParams& P, const int32_t* list, const int32_t* count, int grid, hipStream_t stream) {
+    if (P.pairEnd)
+        hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, true, true, FR>), dim3(grid), dim3(NT), 0, stream, P, list,
+                           count);
+    else
+        hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, false, true, FR>), dim3(grid), dim3(NT), 0, stream, P, list,
+                           count);
+}
+
+hipError_t launchGeodesicLdsLarge(const GeoParams& Q, int tier, int32_t* list, int32_t* count, SpillRecord* spill, hipStream_t stream) {
+    if (Q.nSrc <= 0) return hipSuccess;
+    GeoParams P = Q;
+    hipLaunchKernelGGL(listOverflowedKernel, dim3((P.nSrc + 255) / 256), dim3(256), 0, stream, P.nSrc, (const int*)P.status, list, count);
+    static int cus = 0;
+    if (cus == 0 && (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || cus <= 0)) cus = 256;
+    // the spill queue per workgroup: kTierSpillBytes over the grid (8,192 records for 512 workgroups, 16,384 for 256)
+    const int grid = std::min(tier == 1 ? 2 * cus : cus, 512);
+    P.spill = spill;
+    P.spillPerBlock = 1;
+    P.spillCap = (int)(kTierSpillBytes / sizeof(SpillRecord) / grid);
+    if (tier == 1)  // two 256-lane workgroups per CU, 78 KB of LDS each: up to 1,024 faces, 640 vertices, 256 targets
+        launchLdsList<1024, 640, 576, 256, 2048, 1024, 256, 2, 0>(P, list, count, grid, stream);
+    else  // one 512-lane workgroup per CU, 117 KB of LDS: up to 1,536 faces, 896 vertices, 256 targets; it searches the
+          // patch itself when the patch kernel could not list it (more faces than a list holds)
+        launchLdsList<1536, 896, 896, 256, 4096, 2048, 512, 1, 512>(P, list, count, grid, stream);
+    return hipGetLastError();
 }
 
 hipError_t launchGeodesicLds(const GeoParams& P, int variant, hipStream_t stream) {
