@@ -814,7 +814,8 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
     int h = 0;  // every lane advances its own copy of the head
     for (int step = 0;; ++step) {
         const int t = sh.tail;
-        const int sp = sh.spillTail - sh.spillHead;  // spilled records waiting
+        const int s0 = sh.spillHead;
+        const int sp = sh.spillTail - s0;  // spilled records waiting
         const bool ovf = sh.overflow != 0;
         ldsBarrier();  // every wave has read the tail and the flags before any wave appends or sets one
         if (ovf || (h >= t && sp == 0)) break;
@@ -822,41 +823,37 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
             if (lane == 0) atomicOr(&sh.overflow, OVF_STEPS);
             break;
         }
-        const int room = min((int)blockDim.x, T.qcap) - (t - h);  // free lanes of the next batch (and free ring slots)
-        if (sp > 0 && room > 0) {  // the oldest spilled records fill the batch before it runs
-            __syncthreads();  // a full barrier: every wave's spill stores are visible before they are read
-            const int m = min(sp, room);
-            const int s0 = sh.spillHead;
-            for (int i = lane; i < m; i += blockDim.x) {
-                const SpillRecord r = spill[(s0 + i) % P.spillCap];
-                const int q = (t + i) % T.qcap;
-                T.qb0[q] = r.b0;
-                T.qb1[q] = r.b1;
-                T.qsx[q] = r.sx;
-                T.qsy[q] = r.sy;
-                T.qsg[q] = r.sg;
-                T.qfk[q] = r.fk;
-                T.qt[q] = make_float2(r.tx, r.ty);
-            }
-            __syncthreads();
-            if (lane == 0) {
-                sh.tail = t + m;
-                sh.spillHead = s0 + m;
-            }
-            __syncthreads();
-            continue;
-        }
-        const int nb = min((int)blockDim.x, t - h);
+        // the batch: the ring's items, then the oldest spilled records on the lanes left (their stores were made
+        // visible by the full barrier that ended the step in which they were spilled)
+        const int nRing = min((int)blockDim.x, t - h);
+        const int take = min((int)blockDim.x - nRing, sp);
+        const int nb = nRing + take;
+        if (lane == 0 && take > 0) sh.spillHead = s0 + take;  // appends read the head only after the next barrier
         const bool mine = lane < nb;
-        const int q = (h + lane) % T.qcap;
         int fk = -1, kind = 0, g = -1, fs = 0, k = 0;  // kind: 0 window, 1 vertex event, 2 vertex-face item
         double b0 = 0, b1 = 0, sx = 0, sy = 0, sg = 0, L = 0, x2 = 0, y2 = 0;
         double cf[6] = {0, 0, 0, 0, 0, 0};
         float2 rot = make_float2(0.f, 0.f);
         if (mine) {
-            fk = T.qfk[q];
-            sg = T.qsg[q];
-            rot = T.qt[q];
+            if (lane < nRing) {
+                const int q = (h + lane) % T.qcap;
+                fk = T.qfk[q];
+                sg = T.qsg[q];
+                rot = T.qt[q];
+                b0 = T.qb0[q];
+                b1 = T.qb1[q];
+                sx = T.qsx[q];
+                sy = T.qsy[q];
+            } else if (spill) {  // (take > 0 only with a spill queue)
+                const SpillRecord r = spill[(s0 + lane - nRing) % P.spillCap];
+                fk = r.fk;
+                sg = r.sg;
+                rot = make_float2(r.tx, r.ty);
+                b0 = r.b0;
+                b1 = r.b1;
+                sx = r.sx;
+                sy = r.sy;
+            }
             if (fk >= kVertexFace) {
                 kind = 2;
                 fs = (fk & (kVertexFace - 1)) >> 2;
@@ -866,10 +863,6 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
                 kind = 0;
                 fs = fk >> 3;
                 k = (fk >> 1) & 3;
-                b0 = T.qb0[q];
-                b1 = T.qb1[q];
-                sx = T.qsx[q];
-                sy = T.qsy[q];
             } else {
                 kind = 1;
             }
@@ -883,9 +876,9 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
                     for (int i = 0; i < 6; ++i) cf[i] = M.fchild[18 * g + 6 * k + i];
             }
         }
-        // once every lane holds its record, the ring slots [h, h + nb) are free for this step's appends
+        // once every lane holds its record, the ring slots [h, h + nRing) are free for this step's appends
         ldsBarrier();
-        C.qBase = h + nb;
+        C.qBase = h + nRing;
         VCand cP0 = noCand(), cP1 = noCand(), cP2 = noCand();
         bool live = false;
         if (mine) {
@@ -925,8 +918,8 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
             finishVertexCand(C, cP1);
             finishVertexCand(C, cP2);
         }
-        ldsBarrier();
-        h += nb;
+        __syncthreads();  // a full barrier: this step's spill stores are visible to the next batches
+        h += nRing;
         if (lane == 0) {
             sh.nSteps += 1;
             sh.nItems += nb;
