@@ -22,7 +22,7 @@ constexpr int kXcds = 8;  // accelerator complex dies of an MI355X, each with it
 // patch kernel: hash slots per source, capacity of the written face list, and of one breadth-first level
 constexpr int kPatchHashCap = 1024;
 constexpr int kPatchListCap = 768;
-constexpr int kPatchFrontCap = 512;
+constexpr int kPatchFrontCap = 256;
 
 // reasons a source exceeded the capacities of a kernel variant (bits of the per-source status)
 enum OverflowReason : int { OVF_FACES = 1, OVF_VERTS = 2, OVF_QUEUE = 4, OVF_TARGETS = 8, OVF_STEPS = 16, OVF_FRONTIER = 32 };

@@ -51,6 +51,7 @@ __device__ inline uint32_t hashOf(uint32_t key) { return (key * 2654435761u) >> 
 // vertices are numbered 0, 1, ... in insertion order (compact slots); hash capacities are powers of two.
 template <typename IT>
 struct Tables {
+    static constexpr bool kSlots = true;  // the face hash also maps ids to compact slots
     int32_t* hkey;   // face hash: global face id or -1
     IT* hslot;       // face hash: compact slot of the key
     int32_t* vhkey;  // vertex hash: global vertex id or -1
@@ -161,7 +162,7 @@ __device__ void faceInsert(const TT& T, Shared& sh, int g, bool& fresh) {
                     atomicOr(&sh.overflow, OVF_FACES);
                     return;
                 }
-                T.hslot[h] = slot;
+                if constexpr (TT::kSlots) T.hslot[h] = slot;
                 T.fglob[slot] = g;
                 fresh = true;
                 return;
@@ -1265,16 +1266,15 @@ __global__ __launch_bounds__(64) void geodesicForceScratch(GeoParams P, ScratchL
 // Patch kernel: the reference's submesh face set of every source, one wavefront and ~13 KB of LDS per source (so
 // a dozen sources per CU hide the search's memory latency), written as a compact list (count -1: more faces than the
 // list holds, or a breadth-first level wider than its buffer).
-struct FaceSet {
+struct FaceSet {  // membership only: the list's order numbers the faces, no id -> slot map is kept
+    static constexpr bool kSlots = false;
     int32_t* hkey;
-    int16_t* hslot;
     int32_t* fglob;
     int hmask, fcap;
 };
 
 __global__ __launch_bounds__(64) void patchKernel(GeoParams P, int32_t* list, int32_t* count, int stride) {
     __shared__ int32_t hkey[kPatchHashCap];
-    __shared__ int16_t hslot[kPatchHashCap];
     __shared__ int32_t fglob[kPatchListCap];
     __shared__ int32_t cur[kPatchFrontCap], nxt[kPatchFrontCap];
     __shared__ Shared sh;
@@ -1284,7 +1284,7 @@ __global__ __launch_bounds__(64) void patchKernel(GeoParams P, int32_t* list, in
     const int pi = P.srcIdx ? P.srcIdx[src] : src;
     const int sf = P.pface[pi];
     const int tb = P.toff[src], nT = P.toff[src + 1] - tb;
-    FaceSet T{hkey, hslot, fglob, kPatchHashCap - 1, min(stride, kPatchListCap)};
+    FaceSet T{hkey, fglob, kPatchHashCap - 1, min(stride, kPatchListCap)};
     for (int i = lane; i < kPatchHashCap; i += blockDim.x) hkey[i] = -1;
     if (lane == 0) {
         sh.nFaces = sh.overflow = sh.nCur = sh.nNext = sh.goals = 0;
