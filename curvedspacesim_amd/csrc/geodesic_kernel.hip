@@ -1391,8 +1391,14 @@ hipError_t launchGeodesicLdsLarge(const GeoParams& Q, int tier, int32_t* list, i
     if (Q.nSrc <= 0) return hipSuccess;
     GeoParams P = Q;
     hipLaunchKernelGGL(listOverflowedKernel, dim3((P.nSrc + 255) / 256), dim3(256), 0, stream, P.nSrc, (const int*)P.status, list, count);
-    static int cus = 0;
-    if (cus == 0 && (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || cus <= 0)) cus = 256;
+    // CUs of the current device (cached per device)
+    static int cusOf[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (cusOf[dev] <= 0 && (hipDeviceGetAttribute(&cusOf[dev], hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                            cusOf[dev] <= 0))
+        cusOf[dev] = 256;
+    const int cus = cusOf[dev];
     // the spill queue per workgroup: kTierSpillBytes over the grid (8,192 records for 512 workgroups, 16,384 for 256)
     const int grid = std::min(tier == 1 ? 2 * cus : cus, 512);
     P.spill = spill;
