@@ -57,6 +57,7 @@ def csg():
     L.csg_mesh_upload.argtypes = [VP, C.c_int32, _f64p, C.c_int32, _i32p]
     L.csg_mesh_info.argtypes = [VP, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_double), _f64p, _f64p]
     L.csg_set_submeshing.argtypes = [VP, C.c_int32, C.c_double]
+    L.csg_set_boundary.argtypes = [VP, C.c_int32]
     L.csg_distance_batch.argtypes = [VP, C.c_int32, _i32p, _f64p, _i32p, _i32p, _f64p, VP, VP, VP, VP, VP]
     L.csg_pair_forces.argtypes = [VP, C.c_int32, _i32p, _f64p, C.c_int32, VP, _i32p, _i32p, VP, C.c_double,
                                   C.POINTER(csg_potential), C.c_int32, C.c_double, VP, VP, VP, VP, VP, VP]
@@ -212,6 +213,16 @@ class Context:
         lo, hi = np.zeros(3), np.zeros(3)
         self._check(self.L.csg_mesh_info(self.h, C.byref(nv), C.byref(nf), C.byref(area), lo, hi), "csg_mesh_info")
         return dict(nV=nv.value, nF=nf.value, area=area.value, lo=lo, hi=hi)
+
+    BOUNDARY_MODES = {"closed": 0, "absorbing": 1, "tangential": 2}
+
+    def set_boundary(self, mode):
+        """What every walk of this context does at the border of an open mesh: "closed" (an error, the closed
+        triangulatedMeshSpace), "absorbing" (absorbingOpenMeshSpace: stop on the border) or "tangential"
+        (tangentialOpenMeshSpace: slide along it)."""
+        if mode not in self.BOUNDARY_MODES:
+            raise ValueError(f"unknown boundary mode {mode!r}")
+        self._check(self.L.csg_set_boundary(self.h, self.BOUNDARY_MODES[mode]), "csg_set_boundary")
 
     def set_submeshing(self, enabled, maximum_distance=1.0):
         self._check(self.L.csg_set_submeshing(self.h, int(bool(enabled)), float(maximum_distance)), "csg_set_submeshing")

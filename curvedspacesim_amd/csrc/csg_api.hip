@@ -455,6 +455,7 @@ csg_status csg_transport_batch(csg_ctx* c, int32_t n, int32_t* face, double* bar
     T.vecs = c->tvecs.as<double>();
     T.status = c->tstat.as<int32_t>();
     T.maxCrossings = 100000;
+    T.boundary = c->boundary;
     HIPCHECK(c, csg::launchTransport(T, c->stream));
     std::vector<int32_t> st(n);
     HIPCHECK(c, download(c, face, c->pface, (size_t)n));

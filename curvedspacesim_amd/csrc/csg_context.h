@@ -55,6 +55,7 @@ struct csg_ctx {
     bool haveMesh = false;
     int submeshing = 0;
     double maximumDistance = 0;
+    int boundary = CSG_BOUNDARY_CLOSED;  // walk behaviour at the mesh border
     // per-call staging
     DevBuf pface, pbary, srcIdx, toff, tidx, prad, pdist, pst, pet, pflags, pforces, penergy, status, list, plist, pcount, spill;
     DevBuf tdisp, tvecs, tstat;

@@ -614,6 +614,7 @@ static csg_status sysMove(csg_ctx* c, SystemState* S) {
     T.vecs = S->vel.as<double>();
     T.status = S->tstatus.as<int32_t>();
     T.maxCrossings = 100000;
+    T.boundary = c->boundary;
     {
         MarkScope mark(S, c->stream, MARK_TRANSPORT);
         HIPCHECK(c, launchTransport(T, c->stream));
@@ -847,6 +848,7 @@ csg_status csg_system_move(csg_ctx* c, const double* disp, int32_t transportVelo
     T.vecs = S->vel.as<double>();
     T.status = S->tstatus.as<int32_t>();
     T.maxCrossings = 100000;
+    T.boundary = c->boundary;
     HIPCHECK(c, launchTransport(T, c->stream));
     hipLaunchKernelGGL(kCountNonzero, dim3(blocksFor(S->N)), dim3(kBlock), 0, c->stream, S->N, S->tstatus.as<int>(), S->counters.as<int>(),
                        (int)CNT_WALK_FAIL, true);
@@ -1021,6 +1023,15 @@ csg_status csg_system_autotune(csg_ctx* c, int32_t* chosen, double* ms) {
     S->ldsVariant = best;
     S->profiling = prof;
     if (chosen) *chosen = best;
+    return CSG_OK;
+}
+
+csg_status csg_set_boundary(csg_ctx* c, int32_t mode) {
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (mode != CSG_BOUNDARY_CLOSED && mode != CSG_BOUNDARY_ABSORBING && mode != CSG_BOUNDARY_TANGENTIAL)
+        return fail(c, CSG_ERR_ARGUMENT, "unknown boundary mode");
+    c->boundary = mode;
+    if (SystemState* S = sysOf(c)) dropGraph(S);  // a captured step holds the walk's parameters
     return CSG_OK;
 }
 

@@ -1,4 +1,6 @@
-// gpuMeshSpace: a closed triangulated mesh space whose geodesics and walks run on the GPU through libcsg.so.
+// gpuMeshSpace: a triangulated mesh space whose geodesics and walks run on the GPU through libcsg.so. As the reference's
+// triangulatedMeshSpace it treats the mesh as closed (a walk that meets a border fails); the open-mesh subclasses below
+// give the reference's absorbing and tangential border conditions.
 // It keeps the public interface of triangulatedMeshSpace (src/models/triangulatedMeshSpace.h:21-140) for the calls on
 // the geodesic force path, and adds batched forms that the batched model uses. Failures throw std::runtime_error
 // (the reference prints an error and throws std::exception, inc/debuggingHelp.h:19-31).
@@ -54,6 +56,10 @@ public:
     const std::vector<double>& vertexCoordinates() const { return xyz; }
     const std::vector<int32_t>& faceCorners() const { return tri; }
 
+protected:
+    // CSG_BOUNDARY_*: what every walk of this space does at the mesh border
+    void setBoundary(int mode) { check(csg_set_boundary(ctx, mode), "csg_set_boundary"); }
+
 private:
     void check(csg_status s, const char* what) const;
     vector3 faceNormal(int f) const;
@@ -64,4 +70,18 @@ private:
     double area = 0;
     bool submeshing = false;
     double maxDistance = 0;
+};
+
+// absorbingOpenMeshSpace (src/models/absorbingOpenMeshSpace.h:8-30): a particle that meets the border stops there, and
+// transported vectors lose their component pointing over it
+class gpuAbsorbingOpenMeshSpace : public gpuMeshSpace {
+public:
+    explicit gpuAbsorbingOpenMeshSpace(int device = 0) : gpuMeshSpace(device) { setBoundary(CSG_BOUNDARY_ABSORBING); }
+};
+
+// tangentialOpenMeshSpace (src/models/tangentialOpenMeshSpace.h:8-33): a particle that meets the border slides along it
+// for the rest of the step, and transported vectors lose their component pointing over it
+class gpuTangentialOpenMeshSpace : public gpuMeshSpace {
+public:
+    explicit gpuTangentialOpenMeshSpace(int device = 0) : gpuMeshSpace(device) { setBoundary(CSG_BOUNDARY_TANGENTIAL); }
 };

@@ -1,6 +1,7 @@
 // Launch interface of the batched displacement walk (transport_kernel.hip).
 #pragma once
 
+#include "../../include/csg.h"
 #include "csg_common.h"
 
 namespace csg {
@@ -9,7 +10,8 @@ constexpr int kMaxTransportVectors = 2;  // the reference transports at most for
 
 enum TransportError : int {
     TRANSPORT_ERR_NAN = -4,          // reference: checkBaryNan throws (src/utility/meshUtilities.cpp:184-195)
-    TRANSPORT_ERR_NO_FACE = -7,      // throughVertex found no face (src/models/triangulatedMeshSpace.cpp:404-405)
+    TRANSPORT_ERR_NO_FACE = -7,      // throughVertex found no face (src/models/triangulatedMeshSpace.cpp:404-405), or no
+                                     // other face at a border vertex (src/models/openMeshSpace.cpp:56-57)
     TRANSPORT_ERR_TOO_MANY = -8,     // more face crossings than the configured maximum
     TRANSPORT_ERR_BORDER = -9,       // border edge or vertex on a closed surface (:522-523, :547-548)
     TRANSPORT_ERR_NO_CROSSING = -10, // neither 1 nor 2 crossings (:534-541)
@@ -29,6 +31,7 @@ struct TransportParams {
     double* vecs;    // in/out, 3 * nVec per particle
     int32_t* status; // optional: crossings (>= 0) or a TransportError; a failed particle keeps its input state
     int maxCrossings;
+    int boundary;    // CSG_BOUNDARY_*: what a walk does at a border edge or vertex of an open mesh
 };
 
 hipError_t launchTransport(const TransportParams& P, hipStream_t stream);

@@ -7,10 +7,27 @@
 // crossing keeps the pre-crossing remaining length and rotates transported vectors only when the normals point into
 // opposite half-spaces; the crossing vertex is the first vertex of the first edge found. One deliberate difference:
 // a rotation about a zero-length axis (coplanar neighbours) is skipped where the reference produces NaN and throws.
+//
+// Open meshes: with an absorbing or tangential boundary the walk follows openMeshSpace::transportParticleAndVectors
+// (src/models/openMeshSpace.cpp:114-238). At a border edge the absorbing handler (absorbingOpenMeshSpace.cpp:2-25) stops
+// the particle on the edge and the tangential one (tangentialOpenMeshSpace.cpp:3-39) turns the remaining displacement
+// along the edge; at a border vertex (openMeshSpace.cpp:3-96, absorbingOpenMeshSpace.cpp:27-50,
+// tangentialOpenMeshSpace.cpp:41-64) the particle stops on the vertex or turns along the heading of largest overlap. As in
+// the reference, the handlers see the displacement measured from the pre-crossing source, and after a border vertex the
+// corner copy and the face normal still describe the previous face (the reference passes both by value). The candidate
+// headings at a border vertex are the edges to the other corners of every incident face except the current one, faces in
+// increasing index and corners in face order, first largest overlap winning (the reference pairs neighbours and faces
+// through CGAL's stored vertex halfedges, an order that depends on how the mesh was built).
 #include <math.h>
 
 #include "csg_common.h"
 #include "transport_kernel.h"
+
+// No fused multiply-adds are formed from separate products and sums in this file: the walk takes decisions on exact
+// zeros and signs (a displacement along a border edge has an exactly zero normal component, so the tangential handler
+// stops at a convex corner; barycentric signs decide the crossed edge), and separately rounded operations make those
+// decisions the same as in the reference's IEEE arithmetic.
+#pragma clang fp contract(off)
 
 namespace csg {
 
@@ -26,7 +43,11 @@ __device__ inline D3 mul(D3 a, double s) { return mk(a.x * s, a.y * s, a.z * s);
 __device__ inline double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ inline D3 cross(D3 a, D3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
 __device__ inline double len(D3 a) { return sqrt(dot(a, a)); }
-__device__ inline D3 unit(D3 a) { return mul(a, 1.0 / len(a)); }
+
+__device__ inline D3 divd(D3 a, double s) { return mk(a.x / s, a.y / s, a.z / s); }
+// the reference's normalize (src/utility/functionUtilities.cpp:6): components divided by the length
+__device__ inline D3 unit(D3 a) { return divd(a, len(a)); }
+__device__ inline D3 normalized(D3 a) { return unit(a); }
 
 __device__ inline D3 vert(const DeviceMesh& M, int v) { return mk(M.vx[3 * v], M.vx[3 * v + 1], M.vx[3 * v + 2]); }
 __device__ inline D3 corner(const DeviceMesh& M, int f, int k) { return vert(M, M.fv[3 * f + k]); }
@@ -70,7 +91,7 @@ __device__ inline void clampNearZero(double* b) {
 }
 
 __device__ inline D3 rotateAbout(D3 p, D3 base, D3 axisDir, double ang) {
-    D3 a = mul(axisDir, 1.0 / len(axisDir));
+    D3 a = divd(axisDir, len(axisDir));
     D3 q = sub(p, base);
     double ad = dot(a, q);
     double s = sin(ang), c = cos(ang);
@@ -104,6 +125,46 @@ __device__ inline int across(const DeviceMesh& M, int f, int a, int b) {
         if ((u == a && v == b) || (u == b && v == a)) return M.fadj[3 * f + k] < 0 ? -1 : (M.fadj[3 * f + k] >> 2);
     }
     return -1;
+}
+
+// projectVectorsIfOverBoundary (src/models/triangulatedMeshSpace.cpp:411-426): a vector pointing over the border line
+// (orthogonal: the line's normal direction, inward: towards the mesh) loses its component along that normal
+__device__ inline void projectIfOverBoundary(D3* vec, int nv, D3 orthogonal, D3 inward) {
+    const bool pointsOut = dot(orthogonal, inward) < 0;
+    for (int j = 0; j < nv; ++j) {
+        const bool alongPerp = dot(vec[j], orthogonal) > 0;
+        if ((pointsOut && alongPerp) || (!pointsOut && !alongPerp)) {
+            D3 dh = normalized(orthogonal);
+            vec[j] = sub(vec[j], mul(dh, dot(vec[j], dh)));
+        }
+    }
+}
+
+// getBoundaryVertexHeading (src/models/openMeshSpace.cpp:3-70) with the candidate rule in the header comment: returns
+// the next face (or TRANSPORT_ERR_NO_FACE) and writes the unit heading and the vertex it points to
+__device__ int borderVertexHeading(const DeviceMesh& M, int v, int sourceFace, D3 disp, D3& heading, int& towards) {
+    const D3 dn = normalized(disp);
+    const D3 pv = vert(M, v);
+    double best = -1;
+    int next = sourceFace;
+    towards = v;
+    heading = mk(0, 0, 0);
+    for (int q = M.vfoff[v]; q < M.vfoff[v + 1]; ++q) {
+        const int g = M.vflist[q] >> 2, k = M.vflist[q] & 3;
+        if (g == sourceFace) continue;
+        for (int j = 1; j <= 2; ++j) {
+            const int w = M.fv[3 * g + (k + j) % 3];
+            const D3 out = normalized(sub(vert(M, w), pv));
+            const double ov = dot(out, dn);
+            if (ov > best) {
+                heading = out;
+                best = ov;
+                next = g;
+                towards = w;
+            }
+        }
+    }
+    return next == sourceFace ? TRANSPORT_ERR_NO_FACE : next;
 }
 
 constexpr int kMaxRing = 32;
@@ -201,7 +262,7 @@ __global__ __launch_bounds__(256) void transportKernel(TransportParams P) {
         if (tgtB[0] >= 0 && tgtB[1] >= 0 && tgtB[2] >= 0) break;
         int vl[3] = {M.fv[3 * curFace], M.fv[3 * curFace + 1], M.fv[3 * curFace + 2]};
         double hit[3] = {0, 0, 0}, h[3];
-        int nHits = 0, firstA = -1, firstB = -1;
+        int nHits = 0, firstA = -1, firstB = -1, firstC = 0;
         const int order[3] = {2, 0, 1};  // V1V2, V2V3, V3V1
         for (int oi = 0; oi < 3; ++oi) {
             int c = order[oi];
@@ -212,6 +273,7 @@ __global__ __launch_bounds__(256) void transportKernel(TransportParams P) {
                 if (nHits == 0) {
                     firstA = a;
                     firstB = b;
+                    firstC = c;
                 }
                 ++nHits;
             }
@@ -222,12 +284,45 @@ __global__ __launch_bounds__(256) void transportKernel(TransportParams P) {
         for (int q = 0; q < 3; ++q) srcB[q] = hit[q];
         src = edgePt;
         ++crossings;
-        if (nHits == 2) {
-            int vI = firstA;
-            if (M.vflag[vI] & 2) {
+        if (nHits == 2 && (M.vflag[firstA] & 2)) {
+            // border vertex (openMeshSpace.cpp:186-192)
+            if (P.boundary == CSG_BOUNDARY_CLOSED) {
                 err = TRANSPORT_ERR_BORDER;
                 break;
             }
+            const int vI = firstA;
+            D3 heading;
+            int towards;
+            const int nf = borderVertexHeading(M, vI, curFace, disp, heading, towards);
+            if (nf < 0) {
+                err = nf;
+                break;
+            }
+            curFace = nf;
+            const D3 pv = vert(M, vI);
+            const D3 g0 = corner(M, curFace, 0), g1 = corner(M, curFace, 1), g2 = corner(M, curFace, 2);
+            baryOf(g0, g1, g2, pv, srcB);
+            if (nv > 0) {
+                // projectVectorsForBoundaryVertex (openMeshSpace.cpp:72-96), with the previous face's normal
+                D3 inside = mk(0, 0, 0);
+                for (int k = 0; k < 3; ++k) {
+                    const int sv = M.fv[3 * curFace + k];
+                    if (sv != vI && sv != towards) inside = vert(M, sv);
+                }
+                projectIfOverBoundary(vec, nv, normalized(cross(n, heading)), sub(inside, pv));
+            }
+            lastA = lastB = -1;
+            if (P.boundary == CSG_BOUNDARY_ABSORBING) {
+                for (int q = 0; q < 3; ++q) tgtB[q] = srcB[q];
+                break;
+            }
+            const D3 dh = normalized(heading);
+            disp = mul(dh, dot(disp, dh));
+            target = add(pointOf(g0, g1, g2, srcB), disp);
+            continue;
+        }
+        if (nHits == 2) {
+            int vI = firstA;
             D3 heading;
             int tf = throughVertex(M, vI, toI, curFace, heading);
             if (tf < 0) {
@@ -259,8 +354,30 @@ __global__ __launch_bounds__(256) void transportKernel(TransportParams P) {
         }
         int nf = across(M, curFace, firstA, firstB);
         if (nf < 0) {
-            err = TRANSPORT_ERR_BORDER;
-            break;
+            // border edge (openMeshSpace.cpp:217-222)
+            if (P.boundary == CSG_BOUNDARY_CLOSED) {
+                err = TRANSPORT_ERR_BORDER;
+                break;
+            }
+            const D3 ev1 = vert(M, firstA), ev2 = vert(M, firstB);
+            const D3 at = pointOf(f0, f1, f2, srcB);
+            const D3 inward = sub(firstC == 0 ? cc0 : (firstC == 1 ? cc1 : cc2), at);
+            lastA = lastB = -1;
+            if (P.boundary == CSG_BOUNDARY_ABSORBING) {
+                if (nv > 0) projectIfOverBoundary(vec, nv, normalized(cross(normalOf(M, curFace), sub(ev2, ev1))), inward);
+                for (int q = 0; q < 3; ++q) tgtB[q] = srcB[q];
+                break;
+            }
+            const D3 fwd = normalized(sub(ev2, ev1)), bwd = normalized(sub(ev1, ev2));
+            const D3 dn = normalized(disp);
+            const double fd = dot(dn, fwd), bd = dot(dn, bwd);
+            const double l = len(disp);
+            projectIfOverBoundary(vec, nv, normalized(cross(n, sub(ev2, ev1))), inward);
+            const bool stop = fd <= 0 && bd <= 0;
+            disp = fd > bd ? mul(fwd, fd * l) : mul(bwd, bd * l);
+            target = add(at, disp);
+            if (stop) break;  // the barycentrics computed at the top of this iteration stand
+            continue;
         }
         src = pointOf(f0, f1, f2, hit);
         D3 tn = normalOf(M, nf);
@@ -269,7 +386,7 @@ __global__ __launch_bounds__(256) void transportKernel(TransportParams P) {
         D3 ax = cross(n, tn);
         double axl = len(ax);
         if (nd2 < 1 && axl > 0) {
-            ax = mul(ax, 1.0 / axl);
+            ax = divd(ax, axl);
             target = rotateAbout(target, src, ax, ang);
             disp = sub(target, src);
             for (int j = 0; j < nv; ++j) vec[j] = sub(rotateAbout(add(src, vec[j]), src, ax, ang), src);

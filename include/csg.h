@@ -89,6 +89,18 @@ csg_status csg_pair_forces(csg_ctx* ctx, int32_t nP, const int32_t* face, const 
                            const csg_potential* pot, int32_t useSentinel, double sentinelDistance, double* dist, float* startTan,
                            float* endTan, uint32_t* flags, double* forces, double* energy);
 
+/* --- open meshes: the space class decides what a walk does at the mesh border ------------------------------------
+ * CSG_BOUNDARY_CLOSED: triangulatedMeshSpace (a border edge or vertex is a walk error, triangulatedMeshSpace.cpp:522-548);
+ * CSG_BOUNDARY_ABSORBING: absorbingOpenMeshSpace (src/models/absorbingOpenMeshSpace.cpp:2-50), the particle stops where
+ * it meets the border; CSG_BOUNDARY_TANGENTIAL: tangentialOpenMeshSpace (src/models/tangentialOpenMeshSpace.cpp:3-64), it
+ * slides along the border. Both follow openMeshSpace::transportParticleAndVectors (src/models/openMeshSpace.cpp:114-238)
+ * and project transported vectors that point over the border. Applies to csg_transport_batch and to every walk of the
+ * particle system; default CSG_BOUNDARY_CLOSED. */
+#define CSG_BOUNDARY_CLOSED 0
+#define CSG_BOUNDARY_ABSORBING 1
+#define CSG_BOUNDARY_TANGENTIAL 2
+csg_status csg_set_boundary(csg_ctx* ctx, int32_t mode);
+
 /* --- triangulatedMeshSpace::transportParticleAndVectors (src/models/triangulatedMeshSpace.cpp:448-564), batched -----
  * n particles; face, bary (3 each) and disp (3 each) are in/out as in the reference; vecs holds nVec (0..2)
  * transported vectors per particle (3 doubles each), in/out. status (nullable) receives the number of face crossings

@@ -7,7 +7,10 @@
  *   orc_geodesic        brute-force unfolding on tiny meshes, closed forms (plane, cube, tetrahedron), symmetry,
  *                       triangle inequality, d >= Euclidean, full mesh = patch for short pairs
  *   orc_patch           hand-worked cases of the reference rule on a small strip mesh and invariants
- *   orc_transport       flat-mesh exact translation, cube unfolding, speed preservation, reversibility
+ *   orc_transport       flat-mesh exact translation, cube unfolding, speed preservation, reversibility; open-mesh
+ *                       border edges (absorbing stop point, tangential slide, vector projection) against closed forms on
+ *                       flat meshes; the border-vertex rule (absorbing: stop at the vertex) likewise; the tangential
+ *                       continuation after a border vertex only through invariants (parity unpinned there)
  *   orc_cell_candidates brute-force O(N^2) Euclidean test and the reference cell order on hand-built grids
  *   orc_pair_force      closed forms of the potentials
  * Functions that only a CGAL build could pin (CGAL's rounding and its choice among tied geodesics) are reported as
@@ -35,6 +38,8 @@ int orc_geodesic(const orc_mesh* m, const int* patchFaces, int nPatch, int srcFa
 int orc_distance(const orc_mesh* m, int submeshing, double maximumDistance, int srcFace, const double* srcBary, int nT, const int* tgtFace,
                  const double* tgtBary, double distanceThreshold, double* dist, double* startTan, double* endTan, int* flags);
 
+/* walk behaviour at the mesh border: 0 closed surface (error), 1 absorbing, 2 tangential */
+int orc_mesh_set_boundary(orc_mesh* m, int mode);
 int orc_transport(const orc_mesh* m, int* face, double* bary, double* disp, int nVec, double* vecs);
 int orc_transport_batch(const orc_mesh* m, int n, int* face, double* bary, double* disp, int nVec, double* vecs, int* status);
 void orc_positions(const orc_mesh* m, int n, const int* face, const double* bary, double* out);

@@ -41,6 +41,9 @@ struct Mesh {
     std::vector<int> vfOff, vfList;  // vertex -> incident faces (CSR), entries are face*4 + local index
     std::vector<char> border;      // vertex touches a border edge of the whole mesh
     std::vector<double> angleSum;  // total face angle at each vertex
+    // what a walk does at the mesh border: 0 closed surface (an error), 1 absorbing, 2 tangential
+    // (absorbingOpenMeshSpace / tangentialOpenMeshSpace, src/models/*OpenMeshSpace.cpp)
+    int boundary = 0;
 
     V3 P(int f, int k) const { return X[F[3 * f + ((k % 3) + 3) % 3]]; }
     int fv(int f, int k) const { return F[3 * f + ((k % 3) + 3) % 3]; }

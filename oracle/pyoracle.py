@@ -45,6 +45,7 @@ def lib(fast=False, path=None):
                                _f64p, _f64p, _f64p, _i32p, C.c_void_p]
     L.orc_distance.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, _f64p, C.c_int, _i32p, _f64p, C.c_double,
                                _f64p, _f64p, _f64p, _i32p]
+    L.orc_mesh_set_boundary.argtypes = [C.c_void_p, C.c_int]
     L.orc_transport.argtypes = [C.c_void_p, C.POINTER(C.c_int), _f64p, _f64p, C.c_int, C.c_void_p]
     L.orc_transport_batch.argtypes = [C.c_void_p, C.c_int, _i32p, _f64p, _f64p, C.c_int, C.c_void_p, _i32p]
     L.orc_positions.argtypes = [C.c_void_p, C.c_int, _i32p, _f64p, _f64p]
@@ -158,6 +159,13 @@ class OracleMesh:
         if rc:
             raise RuntimeError(f"orc_distance failed ({rc}): {self.L.orc_last_error().decode()}")
         return d, st, et, fl
+
+    BOUNDARY_MODES = {"closed": 0, "absorbing": 1, "tangential": 2}
+
+    def set_boundary(self, mode):
+        """Walk behaviour at the mesh border: "closed" (an error), "absorbing" or "tangential"."""
+        if self.L.orc_mesh_set_boundary(self.h, self.BOUNDARY_MODES[mode]):
+            raise ValueError(mode)
 
     def transport(self, face, bary, disp, vecs=None):
         """Walk many particles; returns (face, bary, disp, vecs, status)."""

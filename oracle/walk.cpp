@@ -15,6 +15,25 @@
 //   * the vertex handed to the vertex crossing is the first vertex of the first edge found (:526-528).
 // One deliberate difference: a rotation about a zero-length axis (coplanar neighbouring faces) is skipped, where the
 // reference produces NaN coordinates and throws (:632-633 and src/utility/meshUtilities.cpp:184-195).
+//
+// Open meshes (Mesh::boundary != 0) follow openMeshSpace::transportParticleAndVectors (src/models/openMeshSpace.cpp:
+// 114-238), which calls a border handler where the closed walk would fail:
+//   border edge, absorbing     absorbingOpenMeshSpace::updateAtBoundaryEdge (src/models/absorbingOpenMeshSpace.cpp:2-25)
+//   border edge, tangential    tangentialOpenMeshSpace::updateAtBoundaryEdge (src/models/tangentialOpenMeshSpace.cpp:3-39)
+//   border vertex, both        getBoundaryVertexHeading and projectVectorsForBoundaryVertex
+//                              (src/models/openMeshSpace.cpp:3-96), then the absorbing (:27-50) or tangential (:41-64)
+//                              vertex handler
+//   vector projection          projectVectorsIfOverBoundary (src/models/triangulatedMeshSpace.cpp:411-426),
+//                              projectVectorOntoDirection (src/utility/meshUtilities.cpp:119-123)
+// Kept as the reference has them: the displacement handed to a handler runs from the pre-crossing source (so the
+// remaining length includes the distance already walked to the border); the tangential edge handler stops only when
+// the displacement is orthogonal to the edge and then leaves the barycentrics computed at the top of that iteration;
+// the vertex handlers receive the corner copy and the face normal by value, so after a border-vertex continuation both
+// still describe the previous face.
+// Deliberate difference at a border vertex: the candidate headings are the edges from the vertex to the other corners
+// of every incident face except the current one (faces in increasing index, corners in face order), and the first
+// edge of largest overlap with the displacement wins. The reference pairs neighbours and faces through CGAL's stored
+// vertex halfedges (src/models/openMeshSpace.cpp:24-32), an order that depends on how the mesh was built.
 #include <algorithm>
 #include <cmath>
 #include <set>
@@ -98,6 +117,20 @@ static bool crossEdge(const double* s, const double* t, int c, double* hit) {
     return lam >= 0 && lam <= 1 && mu >= 0 && mu <= 1;
 }
 
+// triangulatedMeshSpace::projectVectorsIfOverBoundary (src/models/triangulatedMeshSpace.cpp:411-426): a vector that
+// points over the border line (orthogonal gives the line's normal direction, inward points into the mesh) loses its
+// component along that normal
+static void projectIfOverBoundary(std::vector<V3>& vecs, const V3& orthogonal, const V3& inward) {
+    bool pointsOut = dot(orthogonal, inward) < 0;
+    for (auto& v : vecs) {
+        bool alongPerp = dot(v, orthogonal) > 0;
+        if ((pointsOut && alongPerp) || (!pointsOut && !alongPerp)) {
+            V3 dh = normalized(orthogonal);
+            v = v - dh * dot(v, dh);
+        }
+    }
+}
+
 struct Walker {
     const Mesh& m;
     explicit Walker(const Mesh& mm) : m(mm) {}
@@ -166,6 +199,33 @@ struct Walker {
         return target;
     }
 
+    // openMeshSpace::getBoundaryVertexHeading (src/models/openMeshSpace.cpp:3-70) with the candidate rule stated at
+    // the top of this file: returns the next face, writes the unit heading and the neighbour vertex it points to
+    int borderVertexHeading(int v, int sourceFace, const V3& disp, V3& heading, int& towards) const {
+        V3 dn = normalized(disp);
+        double best = -1;
+        int next = sourceFace;
+        towards = v;
+        heading = V3(0, 0, 0);
+        for (int q = m.vfOff[v]; q < m.vfOff[v + 1]; ++q) {
+            int g = m.vfList[q] >> 2, k = m.vfList[q] & 3;
+            if (g == sourceFace) continue;
+            for (int j = 1; j <= 2; ++j) {
+                int w = m.fv(g, k + j);
+                V3 out = normalized(m.X[w] - m.X[v]);
+                double ov = dot(out, dn);
+                if (ov > best) {
+                    heading = out;
+                    best = ov;
+                    next = g;
+                    towards = w;
+                }
+            }
+        }
+        if (next == sourceFace) throw OracleError(-7, "no face found for a border vertex crossing");
+        return next;
+    }
+
     // returns number of face-boundary crossings, or throws
     int run(int& face, double* bary, V3& disp, std::vector<V3>& vecs, int maxSteps) {
         int curFace = face;
@@ -211,9 +271,39 @@ struct Walker {
             for (int i = 0; i < 3; ++i) srcB[i] = hit[i];
             src = edgePt;
             ++crossings;
+            if (uninvolved.size() == 2 && m.border[involved[0]]) {
+                int vI = involved[0];
+                if (m.boundary == 0) throw OracleError(-9, "border vertex reached on a closed surface");
+                // openMeshSpace.cpp:186-192 and the vertex handlers (absorbingOpenMeshSpace.cpp:27-50,
+                // tangentialOpenMeshSpace.cpp:41-64)
+                V3 heading;
+                int towards;
+                curFace = borderVertexHeading(vI, curFace, disp, heading, towards);
+                std::array<V3, 3> fresh = {corner(m, curFace, 0), corner(m, curFace, 1), corner(m, curFace, 2)};
+                baryOf(fresh[0], fresh[1], fresh[2], m.X[vI], srcB);
+                if (!vecs.empty()) {
+                    // projectVectorsForBoundaryVertex (openMeshSpace.cpp:72-96), with the caller's (previous) normal
+                    V3 orth = normalized(cross(n, heading));
+                    V3 inside(0, 0, 0);
+                    for (int k = 0; k < 3; ++k) {
+                        int sv = m.fv(curFace, k);
+                        if (sv == vI || sv == towards) continue;
+                        inside = m.X[sv];
+                    }
+                    projectIfOverBoundary(vecs, orth, inside - m.X[vI]);
+                }
+                lastA = lastB = -1;
+                if (m.boundary == 1) {
+                    for (int i = 0; i < 3; ++i) tgtB[i] = srcB[i];
+                    break;
+                }
+                V3 dh = normalized(heading);
+                disp = dh * dot(disp, dh);
+                target = facePoint(m, curFace, srcB) + disp;
+                continue;
+            }
             if (uninvolved.size() == 2) {
                 int vI = involved[0];
-                if (m.border[vI]) throw OracleError(-9, "border vertex reached on a closed surface");
                 V3 heading;
                 int tf = throughVertex(vI, toI, curFace, heading);
                 double len = norm(disp);
@@ -240,7 +330,31 @@ struct Walker {
             if (uninvolved.size() != 1) throw OracleError(-10, "target outside the face but neither 1 nor 2 edge crossings found");
             int a = involved[0], b = involved[1];
             int nf = across(curFace, a, b);
-            if (nf < 0) throw OracleError(-11, "border edge reached on a closed surface");
+            if (nf < 0 && m.boundary == 0) throw OracleError(-11, "border edge reached on a closed surface");
+            if (nf < 0) {
+                // openMeshSpace.cpp:217-222 and the edge handlers
+                V3 ev1 = m.X[a], ev2 = m.X[b];
+                V3 at = facePoint(m, curFace, srcB);
+                V3 inward = corners[uninvolved[0]] - at;
+                lastA = lastB = -1;
+                if (m.boundary == 1) {
+                    // absorbingOpenMeshSpace.cpp:2-25: stop on the edge
+                    if (!vecs.empty()) projectIfOverBoundary(vecs, normalized(cross(m.normal(curFace), ev2 - ev1)), inward);
+                    for (int i = 0; i < 3; ++i) tgtB[i] = srcB[i];
+                    break;
+                }
+                // tangentialOpenMeshSpace.cpp:3-39: slide along the edge
+                V3 fwd = normalized(ev2 - ev1), bwd = normalized(ev1 - ev2);
+                V3 dn = normalized(disp);
+                double fd = dot(dn, fwd), bd = dot(dn, bwd);
+                double len = norm(disp);
+                projectIfOverBoundary(vecs, normalized(cross(n, ev2 - ev1)), inward);
+                bool stop = fd <= 0 && bd <= 0;
+                disp = fd > bd ? fwd * (fd * len) : bwd * (bd * len);
+                target = at + disp;
+                if (stop) break;
+                continue;
+            }
             src = facePoint(m, curFace, hit);
             V3 tn = m.normal(nf);
             double nd2 = dot(n, tn);
@@ -273,6 +387,12 @@ struct Walker {
 using namespace orc;
 
 extern "C" {
+
+int orc_mesh_set_boundary(orc_mesh* om, int mode) {
+    if (mode < 0 || mode > 2) return -1;
+    om->m.boundary = mode;
+    return 0;
+}
 
 // One particle: face/bary/disp are in-out (3 doubles each); vecs holds nVec transported vectors (3 doubles each).
 // Returns the number of face-boundary crossings (>= 0) or a negative error code.

@@ -7,7 +7,7 @@
 //   refapi_scenarios noise <seed>                                                       (no GPU)
 //   refapi_scenarios distance <mesh.off> <submesh> <maxDist> <particles.txt> <source> <threshold>
 //   refapi_scenarios simulate <mesh.off> <particles.txt> <kind> <p0> <p1> <p2> <cellList> <submesh> <maxDist>
-//                             <nve|nvt> <dt> <T> <steps> <mode: host|batched|fused>
+//                             <nve|nvt> <dt> <T> <steps> <mode: host|batched|fused> [closed|absorbing|tangential]
 //
 // points.txt: N, then N lines "x y z". particles.txt: N, then N lines "face b0 b1 b2 v0 v1 v2".
 #include <cstdio>
@@ -170,8 +170,17 @@ int distance(char** a) {
     return 0;
 }
 
-int simulate(char** a) {
-    auto space = std::make_shared<gpuMeshSpace>(0);
+int simulate(int argc, char** a) {
+    // optional last argument: the space class (closed triangulated mesh, or an open mesh with absorbing or tangential
+    // border conditions)
+    const std::string boundary = argc > 16 ? a[16] : "closed";
+    std::shared_ptr<gpuMeshSpace> space;
+    if (boundary == "absorbing")
+        space = std::make_shared<gpuAbsorbingOpenMeshSpace>(0);
+    else if (boundary == "tangential")
+        space = std::make_shared<gpuTangentialOpenMeshSpace>(0);
+    else
+        space = std::make_shared<gpuMeshSpace>(0);
     space->loadMeshFromFile(a[2]);
     Particles p = readParticles(a[3]);
     const int kind = atoi(a[4]);
@@ -251,7 +260,7 @@ int main(int argc, char** argv) {
         if (s == "potentials") return potentials();
         if (s == "noise") return noise(argv);
         if (s == "distance") return distance(argv);
-        if (s == "simulate") return simulate(argv);
+        if (s == "simulate") return simulate(argc, argv);
     } catch (std::exception& e) {
         fprintf(stderr, "ERRORERROR %s\n", e.what());
         return 1;

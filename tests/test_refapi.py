@@ -164,3 +164,39 @@ def test_refapi_simulation_modes_match_oracle(tmp_path, integrator):
         bary = np.array(out["bary"]).reshape(-1, 3)
         np.testing.assert_allclose(om.positions(face, bary), Po, atol=1e-8)
         np.testing.assert_allclose(np.array(out["vel"]).reshape(-1, 3), osys.vel, atol=1e-6 * vs)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("boundary", ["absorbing", "tangential"])
+def test_refapi_open_mesh_simulation_matches_oracle(tmp_path, boundary):
+    """The open-mesh space classes (absorbingOpenMeshSpace / tangentialOpenMeshSpace counterparts) driven like the
+    reference's absorbingSimulation.cpp / tangentialSimulation.cpp: harmonic discs, velocity Verlet on an open height
+    field, in all three model modes, against the oracle's system with the same border condition."""
+    from oracle.pyoracle import OracleMesh, OracleSystem
+    from curvedspacesim_amd import workloads
+    import small_meshes as sm
+
+    X, T = sm.grid(16, 16, 0.25, 0.25, z=lambda x, y: 0.3 * np.sin(1.3 * x) * np.cos(0.9 * y))
+    r = np.random.default_rng(3)
+    f, b = workloads.area_uniform(X, T, 120, r)
+    v = workloads.tangent_velocities(X, T, f, 4.0, r)
+    write_off(tmp_path / "m.off", X, T)
+    write_particles(tmp_path / "p.txt", f, b, v)
+    pot = (0, 1.0 / 0.3 ** 2, 0.3, 0.0)
+    steps, dt = 3, 0.05
+    om = OracleMesh(X, T)
+    om.set_boundary(boundary)
+    osys = OracleSystem(om, f, b, v, pot, 0.3, True, True, 0.3, threads=8)
+    osys.compute_forces()
+    for _ in range(steps):
+        osys.nve_step(dt)
+    Po = om.positions(osys.face, osys.bary)
+    vs = np.abs(osys.vel).max()
+    for mode in ("host", "batched", "fused"):
+        print(f"case refapi simulate open mesh {boundary} {mode}", flush=True)
+        out = run("simulate", tmp_path / "m.off", tmp_path / "p.txt", *pot, 1, 1, 0.3, "nve", dt, 0.0, steps, mode,
+                  boundary)
+        face = np.array(out["face"], np.int32)
+        bary = np.array(out["bary"]).reshape(-1, 3)
+        np.testing.assert_allclose(om.positions(face, bary), Po, atol=1e-8)
+        np.testing.assert_allclose(np.array(out["vel"]).reshape(-1, 3), osys.vel, atol=1e-6 * vs)
