@@ -71,6 +71,10 @@ def csg():
     L.csg_system_nvt.argtypes = [VP, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32]
     L.csg_system_sync.argtypes = [VP]
     L.csg_system_get.argtypes = [VP, VP, VP, VP, VP]
+    L.csg_system_set_state.argtypes = [VP, VP, VP, VP, VP]
+    L.csg_system_thermostat.argtypes = [VP, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double), VP,
+                                        C.POINTER(C.c_double)]
+    L.csg_system_set_thermostat.argtypes = [VP, C.c_int32, C.c_double, C.c_double, VP, C.c_double]
     L.csg_system_status.argtypes = [VP, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                     C.POINTER(C.c_int32), C.POINTER(C.c_double)]
     L.csg_system_pairs.argtypes = [VP, VP, VP, C.c_int64]
@@ -349,6 +353,44 @@ class System:
         self.ctx._check(self.L.csg_system_get(self.ctx.h, _ptr(f), _ptr(b), _ptr(v), _ptr(F)), "csg_system_get")
         return f, b, v, F
 
+    def set_state(self, face=None, bary=None, vel=None, forces=None):
+        """simpleModelDatabase::readState counterpart: replaces the given fields of the device state."""
+        f = None if face is None else np.ascontiguousarray(face, np.int32)
+        b, v, F = (None if a is None else np.ascontiguousarray(a, np.float64).reshape(-1, 3) for a in (bary, vel, forces))
+        for a in (f, b, v, F):
+            if a is not None and len(a) != self.N:
+                raise ValueError("state arrays must hold one entry per particle")
+        self.ctx._check(self.L.csg_system_set_state(self.ctx.h, _ptr(f), _ptr(b), _ptr(v), _ptr(F)), "csg_system_set_state")
+
+    def checkpoint(self, time=0.0):
+        """Everything a resumed run needs, as arrays: the record of the reference's simpleModelDatabase
+        (src/databases/simpleModelDatabase.cpp:37-50: time, R3position, barycentricPosition, faceIndex, velocity, force)
+        and the Nose-Hoover chain, which the reference does not save (nhChain, (M + 1) x 4; nhParams = M, T, tau and the
+        chain's kinetic energy; M = 0 before any NVT step)."""
+        f, b, v, F = self.get()
+        M, T, tau, ke = C.c_int32(), C.c_double(), C.c_double(), C.c_double()
+        h = self.ctx.h
+        self.ctx._check(self.L.csg_system_thermostat(h, C.byref(M), C.byref(T), C.byref(tau), None, C.byref(ke)),
+                        "csg_system_thermostat")
+        chain = np.zeros((M.value + 1 if M.value else 0, 4))
+        if M.value:
+            self.ctx._check(self.L.csg_system_thermostat(h, C.byref(M), C.byref(T), C.byref(tau), _ptr(chain),
+                                                         C.byref(ke)), "csg_system_thermostat")
+        return dict(time=np.array([float(time)]), R3position=self.ctx.positions(f, b), barycentricPosition=b,
+                    faceIndex=f, velocity=v, force=F, nhChain=chain,
+                    nhParams=np.array([M.value, T.value, tau.value, ke.value]))
+
+    def restore(self, ckpt):
+        """Sets the state of a checkpoint (positions, velocities, forces, and the chain when it holds one)."""
+        self.set_state(ckpt["faceIndex"], ckpt["barycentricPosition"], ckpt["velocity"], ckpt["force"])
+        M, T, tau, ke = ckpt["nhParams"]
+        if int(M) > 0:
+            chain = np.ascontiguousarray(ckpt["nhChain"], np.float64)
+            if chain.shape != (int(M) + 1, 4):
+                raise ValueError("Nose-Hoover chain of the wrong shape")
+            self.ctx._check(self.L.csg_system_set_thermostat(self.ctx.h, int(M), float(T), float(tau), _ptr(chain),
+                                                             float(ke)), "csg_system_set_thermostat")
+
     def status(self):
         p, o, u, w, k = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
         self.ctx._check(self.L.csg_system_status(self.ctx.h, C.byref(p), C.byref(o), C.byref(u), C.byref(w), C.byref(k)),
@@ -439,3 +481,20 @@ class System:
         return dict(faces=int(s[0]), verts=int(s[1]), steps=int(s[2]), items=int(s[3]), max_faces=int(m[0]),
                     max_steps=int(m[2]), max_items=int(m[3]), us_patch=s[4] * 0.01, us_propagate=s[5] * 0.01,
                     us_epilogue=s[6] * 0.01)
+
+
+CHECKPOINT_FIELDS = ("time", "R3position", "barycentricPosition", "faceIndex", "velocity", "force", "nhChain", "nhParams")
+
+
+def save_checkpoint(path, ckpt):
+    """Writes a System.checkpoint() to an uncompressed .npz archive of plain arrays."""
+    np.savez(path, **{k: np.asarray(ckpt[k]) for k in CHECKPOINT_FIELDS})
+
+
+def load_checkpoint(path):
+    """Reads a checkpoint written by save_checkpoint (plain arrays only; object arrays are refused)."""
+    with np.load(path, allow_pickle=False) as z:
+        missing = [k for k in CHECKPOINT_FIELDS if k not in z.files]
+        if missing:
+            raise ValueError(f"not a checkpoint: missing {missing}")
+        return {k: z[k].copy() for k in CHECKPOINT_FIELDS}

@@ -909,6 +909,60 @@ csg_status csg_system_get(csg_ctx* c, int32_t* face, double* bary, double* vel, 
     return CSG_OK;
 }
 
+csg_status csg_system_set_state(csg_ctx* c, const int32_t* face, const double* bary, const double* vel, const double* forces) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
+    if (face)
+        for (int i = 0; i < S->N; ++i)
+            if (face[i] < 0 || face[i] >= c->nF) return fail(c, CSG_ERR_ARGUMENT, "particle face out of range");
+    HIPCHECK(c, hipSetDevice(c->device));
+    if (face) HIPCHECK(c, hipMemcpyAsync(S->face.p, face, 4 * (size_t)S->N, hipMemcpyHostToDevice, c->stream));
+    if (bary) HIPCHECK(c, hipMemcpyAsync(S->bary.p, bary, 24 * (size_t)S->N, hipMemcpyHostToDevice, c->stream));
+    if (vel) HIPCHECK(c, hipMemcpyAsync(S->vel.p, vel, 24 * (size_t)S->N, hipMemcpyHostToDevice, c->stream));
+    if (forces) HIPCHECK(c, hipMemcpyAsync(S->force.p, forces, 24 * (size_t)S->N, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return CSG_OK;
+}
+
+csg_status csg_system_thermostat(csg_ctx* c, int32_t* M, double* T, double* tau, double* chain, double* kineticEnergy) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
+    const int m = S->chainReady ? S->M : 0;
+    if (M) *M = m;
+    if (T) *T = S->T;
+    if (tau) *tau = S->tau;
+    double ke[2] = {0, 0};
+    HIPCHECK(c, hipSetDevice(c->device));
+    if (m > 0) {
+        if (chain) HIPCHECK(c, hipMemcpyAsync(chain, S->chain.p, 32 * (size_t)(m + 1), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipMemcpyAsync(ke, S->ke.p, 16, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+    }
+    if (kineticEnergy) *kineticEnergy = ke[0];
+    return CSG_OK;
+}
+
+csg_status csg_system_set_thermostat(csg_ctx* c, int32_t M, double T, double tau, const double* chain, double kineticEnergy) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
+    if (M < 1 || !chain) return fail(c, CSG_ERR_ARGUMENT, "need a chain of length at least 1");
+    HIPCHECK(c, hipSetDevice(c->device));
+    dropGraph(S);
+    const double ke[2] = {kineticEnergy, 1.0};
+    HIPCHECK(c, S->chain.ensure(32 * (size_t)(M + 1)));
+    HIPCHECK(c, hipMemcpyAsync(S->chain.p, chain, 32 * (size_t)(M + 1), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(S->ke.p, ke, 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    S->M = M;
+    S->T = T;
+    S->tau = tau;
+    S->chainReady = true;
+    return CSG_OK;
+}
+
 csg_status csg_system_status(csg_ctx* c, int64_t* pairsLastForce, int32_t* candOverflow, int32_t* unsolved, int32_t* walkFailures,
                              double* kineticEnergy) {
     SystemState* S = sysOf(c);

@@ -138,6 +138,16 @@ csg_status csg_system_nvt(csg_ctx* ctx, double dt, double T, double tau, int32_t
 csg_status csg_system_sync(csg_ctx* ctx);
 /* copies state back (every pointer nullable): faces N, barycentric 3N, velocities 3N, forces 3N */
 csg_status csg_system_get(csg_ctx* ctx, int32_t* face, double* bary, double* vel, double* forces);
+/* checkpoint and resume. simpleModelDatabase::readState (src/databases/simpleModelDatabase.cpp:122-158) restores
+ * positions, velocities and forces: csg_system_set_state does the same from host arrays (every pointer nullable: that
+ * field is kept). The reference saves no thermostat state, so a resumed Nose-Hoover run restarts its chain; here the
+ * chain can be read and set as well, for a resume that continues the same trajectory. csg_system_thermostat: M (0 when
+ * no NVT step ran since csg_system_init), T, tau, chain (4 (M + 1) doubles: position, velocity, acceleration, mass of each
+ * entry; nullable) and the kinetic energy the chain last computed; csg_system_set_thermostat takes the same values, and
+ * the next csg_system_nvt with the same (T, tau, M) continues from them. */
+csg_status csg_system_set_state(csg_ctx* ctx, const int32_t* face, const double* bary, const double* vel, const double* forces);
+csg_status csg_system_thermostat(csg_ctx* ctx, int32_t* M, double* T, double* tau, double* chain, double* kineticEnergy);
+csg_status csg_system_set_thermostat(csg_ctx* ctx, int32_t M, double T, double tau, const double* chain, double kineticEnergy);
 /* counters since csg_system_init: pairs of the last force computation, candidate-capacity overflows, sources left
  * unsolved, failed walks; and the thermostat's kinetic energy (0 before any NVT step) */
 csg_status csg_system_status(csg_ctx* ctx, int64_t* pairsLastForce, int32_t* candOverflow, int32_t* unsolved, int32_t* walkFailures,

@@ -1,0 +1,54 @@
+"""Checkpoint and resume of the device-resident system: a run interrupted by a checkpoint (written to disk, read back
+and restored into a fresh context) continues the same trajectory as an uninterrupted run. The reference restores
+positions, velocities and forces (simpleModelDatabase::readState, src/databases/simpleModelDatabase.cpp:122-158);
+the Nose-Hoover chain is restored as well. Tolerances as for graph replays against direct launches: positions 1e-10,
+velocities 1e-9 of their scale (equal-length windows may be processed in another order)."""
+import numpy as np
+import pytest
+
+import curvedspacesim_amd as csa
+from curvedspacesim_amd import workloads
+from oracle.pyoracle import OracleMesh
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("integrator", ["nve", "nvt"])
+def test_resume_continues_the_trajectory(tmp_path, integrator):
+    w = workloads.config(2, n=2000)
+    print(f"case checkpoint resume {integrator}", flush=True)
+
+    def run(sysm, steps):
+        if integrator == "nve":
+            sysm.nve(w.dt, steps)
+        else:
+            sysm.nvt(w.dt, 0.5, 1.0, 2, steps)
+
+    ca, cb, cc = csa.Context(0), csa.Context(0), csa.Context(0)
+    try:
+        a = workloads.make_system(ca, w)
+        run(a, 6)
+        fa, ba, va, Fa = a.get()
+
+        b = workloads.make_system(cb, w)
+        run(b, 3)
+        ck = b.checkpoint(time=3 * w.dt)
+        csa.save_checkpoint(tmp_path / "ck.npz", ck)
+        cb.close()
+        back = csa.load_checkpoint(tmp_path / "ck.npz")
+        for k in csa._native.CHECKPOINT_FIELDS:
+            np.testing.assert_array_equal(back[k], ck[k])
+        assert (back["nhParams"][0] == 2) == (integrator == "nvt")
+
+        # a fresh context, set up from the same workload, then the checkpoint's state
+        c = workloads.make_system(cc, w)
+        c.restore(back)
+        run(c, 3)
+        fc, bc, vc, Fc = c.get()
+        om = OracleMesh(w.X, w.T)
+        np.testing.assert_allclose(om.positions(fc, bc), om.positions(fa, ba), atol=1e-10)
+        np.testing.assert_allclose(vc, va, atol=1e-9 * np.abs(va).max())
+        assert c.status()["unsolved"] == 0 and c.status()["walk_failures"] == 0
+    finally:
+        ca.close()
+        cc.close()
