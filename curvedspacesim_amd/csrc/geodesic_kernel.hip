@@ -1269,13 +1269,13 @@ __global__ __launch_bounds__(64) void geodesicForceScratch(GeoParams P, ScratchL
 struct FaceSet {  // membership only: the list's order numbers the faces, no id -> slot map is kept
     static constexpr bool kSlots = false;
     int32_t* hkey;
-    int32_t* fglob;
+    int32_t* fglob;  // the output list itself (HBM): faces are written where they are found
     int hmask, fcap;
 };
 
+// One wavefront per source; 6.2 KB of LDS (hash and frontier lists), so that LDS admits 26 sources per CU.
 __global__ __launch_bounds__(64) void patchKernel(GeoParams P, int32_t* list, int32_t* count, int stride) {
     __shared__ int32_t hkey[kPatchHashCap];
-    __shared__ int32_t fglob[kPatchListCap];
     __shared__ int32_t cur[kPatchFrontCap], nxt[kPatchFrontCap];
     __shared__ Shared sh;
     const int src = sourceOfBlock(P);
@@ -1284,7 +1284,8 @@ __global__ __launch_bounds__(64) void patchKernel(GeoParams P, int32_t* list, in
     const int pi = P.srcIdx ? P.srcIdx[src] : src;
     const int sf = P.pface[pi];
     const int tb = P.toff[src], nT = P.toff[src + 1] - tb;
-    FaceSet T{hkey, fglob, kPatchHashCap - 1, min(stride, kPatchListCap)};
+    // slots below fcap <= stride: every write stays inside this source's stride of the list
+    FaceSet T{hkey, list + (size_t)src * stride, kPatchHashCap - 1, min(stride, kPatchListCap)};
     for (int i = lane; i < kPatchHashCap; i += blockDim.x) hkey[i] = -1;
     if (lane == 0) {
         sh.nFaces = sh.overflow = sh.nCur = sh.nNext = sh.goals = 0;
@@ -1293,14 +1294,7 @@ __global__ __launch_bounds__(64) void patchKernel(GeoParams P, int32_t* list, in
     __syncthreads();
     patchSearch(P, T, sh, cur, nxt, kPatchFrontCap, src, sf, tb, nT);
     __syncthreads();
-    if (sh.overflow) {
-        if (lane == 0) count[src] = -1;
-        return;
-    }
-    const int n = sh.nFaces;
-    int32_t* out = list + (size_t)src * stride;
-    for (int i = lane; i < n; i += blockDim.x) out[i] = fglob[i];
-    if (lane == 0) count[src] = n;
+    if (lane == 0) count[src] = sh.overflow ? -1 : sh.nFaces;
 }
 
 // Frames of the two edges a window entering face g through edge k leaves by, per (face, edge): 6 doubles
