@@ -672,11 +672,17 @@ __device__ void patchSearch(const GeoParams& P, const TT& T, Shared& sh, int* cu
 
 // Builds the patch and the per-slot tables. Returns false (with sh.overflow set) when a capacity is exceeded. cur/nxt
 // are breadth-first frontier lists for a source without a prepared face list (nullptr: such a source overflows).
-template <class TT>
+// kIt > 0: a prepared list holds at most kIt faces per lane, whose corner ids and neighbour words are loaded once, all
+// in flight together, and kept in registers from the face set (A) to the per-slot tables (C).
+template <int kIt, class TT>
 __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int* cur, int* nxt, int frontCap, int src, int sf, int tb,
                            int nT) {
     const int lane = threadIdx.x;
+    const int nt = blockDim.x;
     const DeviceMesh& M = P.mesh;
+    constexpr int kR = kIt > 0 ? kIt : 1;
+    int cv[kR][3], ca[kR][3];
+    bool cached = false;
     // targets first, independent of the patch: weights normalised by their sum, and the global face id held in the
     // best-distance word until phase C turns it into a face slot (so these loads overlap the face set's)
     if (nT <= T.tcap)
@@ -700,10 +706,33 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int* cur
         if (listed > T.fcap) {
             if (lane == 0) atomicOr(&sh.overflow, OVF_FACES);
         } else {
-            for (int i = lane; i < listed; i += blockDim.x) {
-                const int g = lst[i];
-                faceInsertAt(T, g, i);
-                for (int k = 0; k < 3; ++k) vertInsert(T, sh, M.fv[3 * g + k]);
+            if constexpr (kIt > 0) {
+                int gl[kR];
+#pragma unroll
+                for (int j = 0; j < kR; ++j) {
+                    const int i = lane + j * nt;
+                    gl[j] = i < listed ? lst[i] : -1;
+                }
+#pragma unroll
+                for (int j = 0; j < kR; ++j)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        cv[j][k] = gl[j] >= 0 ? M.fv[3 * gl[j] + k] : -1;
+                        ca[j][k] = gl[j] >= 0 ? M.fadj[3 * gl[j] + k] : -1;
+                    }
+#pragma unroll
+                for (int j = 0; j < kR; ++j)
+                    if (gl[j] >= 0) {
+                        faceInsertAt(T, gl[j], lane + j * nt);
+                        for (int k = 0; k < 3; ++k) vertInsert(T, sh, cv[j][k]);
+                    }
+                cached = true;
+            } else {
+                for (int i = lane; i < listed; i += nt) {
+                    const int g = lst[i];
+                    faceInsertAt(T, g, i);
+                    for (int k = 0; k < 3; ++k) vertInsert(T, sh, M.fv[3 * g + k]);
+                }
             }
             if (lane == 0) sh.nFaces = listed;
             vertsDone = true;
@@ -724,17 +753,26 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int* cur
     }
     const int nV = sh.nVerts;
     // C. per face slot: corner and neighbour slots, corner counts per vertex; per target: face slot and weights
-    for (int s = lane; s < nF; s += blockDim.x) {
-        const int g = T.fglob[s];
+    auto faceSlotTables = [&](int s, const int* fvg, const int* adj) {
         for (int k = 0; k < 3; ++k) {
-            const int v = vertLookup(T, M.fv[3 * g + k]);
+            const int v = vertLookup(T, fvg[k]);
             T.fcv[3 * s + k] = v;
             atomicAdd(&T.vfo[v], 1);
-            const int a = M.fadj[3 * g + k];
+            const int a = adj[k];
             const int ns = a < 0 ? -1 : faceLookup(T, a >> 2);
             T.fnb[3 * s + k] = ns < 0 ? -1 : ns * 4 + (a & 3);
         }
         T.thead[s] = -1;
+    };
+    if (cached) {
+#pragma unroll
+        for (int j = 0; j < kR; ++j)
+            if (lane + j * nt < nF) faceSlotTables(lane + j * nt, cv[j], ca[j]);
+    } else {
+        for (int s = lane; s < nF; s += nt) {
+            const int g = T.fglob[s];
+            faceSlotTables(s, M.fv + 3 * g, M.fadj + 3 * g);
+        }
     }
     for (int t = lane; t < nT; t += blockDim.x) {
         T.tslot[t] = faceLookup(T, (int)T.tbest[t]);
@@ -768,35 +806,65 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int* cur
     return true;
 }
 
+// Corner k of the source face, and the neighbour across its edge k with the end points and length of that edge in the
+// neighbour's numbering: what the first windows need, loaded before the tables are built so that the loads overlap it.
+struct SourceEdge {
+    double c[3];
+    double o[3], e[3], L;
+    int h, kh;  // neighbour face and its local index of the shared edge; h < 0 on a mesh border
+};
+
+__device__ inline SourceEdge loadSourceEdge(const DeviceMesh& M, int sf, int k) {
+    SourceEdge r{};
+    loadV(M, M.fv[3 * sf + k], r.c);
+    const int a = M.fadj[3 * sf + k];
+    r.h = a < 0 ? -1 : a >> 2;
+    r.kh = a & 3;
+    if (r.h >= 0) {
+        loadV(M, M.fv[3 * r.h + r.kh], r.o);
+        loadV(M, M.fv[3 * r.h + (r.kh + 1) % 3], r.e);
+        r.L = M.fgeo[9 * r.h + 3 * r.kh];
+    }
+    return r;
+}
+
+// lane that holds corner/edge k of the source face (SourceEdge): lanes 64-66, a wavefront other than the one that sets
+// up the source frame
+__device__ inline int sourceEdgeOf(int lane) { return lane - (blockDim.x >= 128 ? 64 : 0); }
+
 template <class TT>
-__device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf, int tb, SpillRecord* spill, double rMax) {
+__device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf, int tb, SpillRecord* spill, double rMax,
+                             const SourceEdge& se) {
     const int lane = threadIdx.x;
+    const int ek = sourceEdgeOf(lane);
     const DeviceMesh& M = P.mesh;
     Ctx<TT> C{P, T, sh, 0, rMax, tb, spill};
     const int ss = sh.srcSlot;
     // corners of the source face are visible directly
-    if (lane < 3) {
-        double p[3];
-        loadV(M, M.fv[3 * sf + lane], p);
+    if (ek >= 0 && ek < 3) {
+        const double* p = se.c;
         double r[3] = {p[0] - sh.S[0], p[1] - sh.S[1], p[2] - sh.S[2]};
         double l = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-        int vs = T.fcv[3 * ss + lane];
+        int vs = T.fcv[3 * ss + ek];
         T.vdist[vs] = dbits(l);  // three distinct corners, one lane each
         T.vtan[vs] = l > 0 ? make_float2((float)((r[0] * sh.t1[0] + r[1] * sh.t1[1] + r[2] * sh.t1[2]) / l),
                                          (float)((r[0] * sh.t2[0] + r[1] * sh.t2[1] + r[2] * sh.t2[2]) / l))
                            : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    if (lane < 3) {
-        int nb = T.fnb[3 * ss + lane];
-        if (nb >= 0) {
-            int hs = nb >> 2, kh = nb & 3, h = T.fglob[hs];
-            double sx, sy, L;
-            pointInFrame(M, h, kh, sh.S, sx, sy, L);
-            double o[3], e[3];
-            loadV(M, M.fv[3 * h + kh], o);
-            loadV(M, M.fv[3 * h + (kh + 1) % 3], e);
+    if (ek >= 0 && ek < 3) {
+        int nb = T.fnb[3 * ss + ek];
+        if (nb >= 0) {  // the neighbour se.h lies in the patch
+            const int hs = nb >> 2, kh = nb & 3;
+            const double L = se.L;
+            const double *o = se.o, *e = se.e;
+            // the source point in the frame of edge kh of the neighbour (pointInFrame)
             double ex[3] = {(e[0] - o[0]) / L, (e[1] - o[1]) / L, (e[2] - o[2]) / L};
+            const double rr[3] = {sh.S[0] - o[0], sh.S[1] - o[1], sh.S[2] - o[2]};
+            const double sx = rr[0] * ex[0] + rr[1] * ex[1] + rr[2] * ex[2];
+            const double c0 = rr[1] * ex[2] - rr[2] * ex[1], c1 = rr[2] * ex[0] - rr[0] * ex[2], c2 = rr[0] * ex[1] - rr[1] * ex[0];
+            double sy = -sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+            if (sy > -1e-14 * L) sy = -1e-14 * L;
             float2 rot = make_float2((float)(ex[0] * sh.t1[0] + ex[1] * sh.t1[1] + ex[2] * sh.t1[2]),
                                      (float)(ex[0] * sh.t2[0] + ex[1] * sh.t2[1] + ex[2] * sh.t2[2]));
             VCand a = noCand(), b = noCand();
@@ -928,7 +996,8 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
     }
 }
 
-template <class TT>
+// kIt: faces per lane of a prepared patch list (0: no register caching, for tables of any size)
+template <int kIt, class TT>
 __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src, int* cur, int* nxt, int frontCap) {
     const int lane = threadIdx.x;
     const DeviceMesh& M = P.mesh;
@@ -950,14 +1019,19 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
         sh.nCur = sh.nNext = sh.goals = 0;
         initSourceFrame(P, sh, pi, sf);
     }
+    SourceEdge se{};
+    {
+        const int ek = sourceEdgeOf(lane);
+        if (ek >= 0 && ek < 3) se = loadSourceEdge(M, sf, ek);
+    }
     SpillRecord* spill = P.spill ? P.spill + (size_t)(P.spillPerBlock ? blockIdx.x : src) * P.spillCap : nullptr;
     __syncthreads();
     bool ok = !sh.overflow;
     __syncthreads();
     const long long c0 = wall_clock64();
-    if (ok) ok = buildPatch(P, T, sh, cur, nxt, frontCap, src, sf, tb, nT);
+    if (ok) ok = buildPatch<kIt>(P, T, sh, cur, nxt, frontCap, src, sf, tb, nT);
     const long long c1 = wall_clock64();
-    if (ok) propagateAll(P, T, sh, sf, tb, spill, rMax);
+    if (ok) propagateAll(P, T, sh, sf, tb, spill, rMax, se);
     __syncthreads();
     const long long c2 = wall_clock64();
     // ---- finalize: paths through the target face's corners, direct visibility, outputs, force
@@ -1190,7 +1264,7 @@ __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P, const i
     if constexpr (kList) {  // persistent over a device list of sources (the larger tables' pass)
         const int n = *count;
         for (int i = blockIdx.x; i < n; i += gridDim.x)
-            solveSource(P, T, sh, list[i], FR > 0 ? front : nullptr, FR > 0 ? front + FR : nullptr, FR);
+            solveSource<(FC + NT - 1) / NT>(P, T, sh, list[i], FR > 0 ? front : nullptr, FR > 0 ? front + FR : nullptr, FR);
         return;
     }
     int src;
@@ -1203,7 +1277,7 @@ __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P, const i
         src = sourceOfBlock(P);
     }
     if (src < 0) return;
-    solveSource(P, T, sh, src, nullptr, nullptr, 0);
+    solveSource<(FC + NT - 1) / NT>(P, T, sh, src, nullptr, nullptr, 0);
 }
 
 // Scratch variant: the same solver with its tables in a per-workgroup slice of an HBM buffer, persistent over a
@@ -1259,7 +1333,7 @@ __global__ __launch_bounds__(64) void geodesicForceScratch(GeoParams P, ScratchL
     for (int i = blockIdx.x; i < nList; i += gridDim.x) {
         const int src = list ? list[i] : i;
         if (redoOnly && P.status[src] == 0) continue;  // solved by the LDS variant
-        solveSource(Q, T, sh, src, cur, nxt, L.fcap);
+        solveSource<0>(Q, T, sh, src, cur, nxt, L.fcap);
     }
 }
 
