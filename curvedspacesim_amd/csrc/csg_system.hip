@@ -152,29 +152,39 @@ __global__ void kCellKeys(Grid g, int N, const double* xyz, int* key, int* cellC
     atomicAdd(&cellCount[c], 1);
 }
 
-// exclusive prefix sum of n ints into out[0..n] (out[n] = total), one workgroup of 1024 lanes
+// exclusive prefix sum of n ints into out[0..n] (out[n] = total), one workgroup of 1024 lanes: each lane sums a
+// contiguous run of ceil(n / 1024) entries, the lanes' sums are scanned across each wavefront with shuffles and across
+// the 16 wavefronts through LDS (two barriers in all), and each lane writes its run
 __global__ __launch_bounds__(1024) void kScanExclusive(const int* in, int* out, int n) {
-    __shared__ int buf[1024];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < n; base += 1024) {
-        int i = base + threadIdx.x;
-        int v = i < n ? in[i] : 0;
-        buf[threadIdx.x] = v;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            int t = threadIdx.x >= o ? buf[threadIdx.x - o] : 0;
-            __syncthreads();
-            buf[threadIdx.x] += t;
-            __syncthreads();
-        }
-        if (i < n) out[i] = carry + buf[threadIdx.x] - v;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry += buf[1023];
-        __syncthreads();
+    __shared__ int wtot[16];
+    const int per = (n + 1023) / 1024;
+    const int b = min((int)threadIdx.x * per, n), e = min(b + per, n);
+    int s = 0;
+    for (int i = b; i < e; ++i) s += in[i];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = s;  // inclusive scan of the lanes' sums within the wavefront
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
     }
-    if (threadIdx.x == 0) out[n] = carry;
+    if (lane == 63) wtot[w] = x;
+    __syncthreads();
+    if (w == 0) {  // inclusive scan of the 16 wavefront totals
+        int t = lane < 16 ? wtot[lane] : 0;
+        for (int o = 1; o < 16; o <<= 1) {
+            const int y = __shfl_up(t, o, 64);
+            if (lane >= o) t += y;
+        }
+        if (lane < 16) wtot[lane] = t;
+    }
+    __syncthreads();
+    int run = x - s + (w > 0 ? wtot[w - 1] : 0);
+    for (int i = b; i < e; ++i) {
+        const int v = in[i];
+        out[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) out[n] = wtot[15];
 }
 
 __global__ void kScatter(int N, const int* key, const int* cellStart, int* cellFill, int* order) {
