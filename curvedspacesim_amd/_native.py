@@ -381,15 +381,16 @@ class System:
                     nhParams=np.array([M.value, T.value, tau.value, ke.value]))
 
     def restore(self, ckpt):
-        """Sets the state of a checkpoint (positions, velocities, forces, and the chain when it holds one)."""
-        self.set_state(ckpt["faceIndex"], ckpt["barycentricPosition"], ckpt["velocity"], ckpt["force"])
+        """Sets the state of a checkpoint: positions, velocities, forces and the Nose-Hoover chain (a checkpoint taken
+        before any NVT step clears the chain, so that the next NVT step starts a new one)."""
         M, T, tau, ke = ckpt["nhParams"]
-        if int(M) > 0:
-            chain = np.ascontiguousarray(ckpt["nhChain"], np.float64)
-            if chain.shape != (int(M) + 1, 4):
-                raise ValueError("Nose-Hoover chain of the wrong shape")
-            self.ctx._check(self.L.csg_system_set_thermostat(self.ctx.h, int(M), float(T), float(tau), _ptr(chain),
-                                                             float(ke)), "csg_system_set_thermostat")
+        chain = np.ascontiguousarray(ckpt["nhChain"], np.float64)
+        if int(M) > 0 and chain.shape != (int(M) + 1, 4):
+            raise ValueError("Nose-Hoover chain of the wrong shape")
+        self.set_state(ckpt["faceIndex"], ckpt["barycentricPosition"], ckpt["velocity"], ckpt["force"])
+        self.ctx._check(self.L.csg_system_set_thermostat(self.ctx.h, int(M), float(T), float(tau),
+                                                         _ptr(chain) if int(M) > 0 else None, float(ke)),
+                        "csg_system_set_thermostat")
 
     def status(self):
         p, o, u, w, k = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()

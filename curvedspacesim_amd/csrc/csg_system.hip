@@ -958,7 +958,11 @@ csg_status csg_system_set_thermostat(csg_ctx* c, int32_t M, double T, double tau
     SystemState* S = sysOf(c);
     if (!c) return CSG_ERR_ARGUMENT;
     if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
-    if (M < 1 || !chain) return fail(c, CSG_ERR_ARGUMENT, "need a chain of length at least 1");
+    if (M == 0) {  // no chain: the next NVT step initialises one, as after csg_system_init
+        S->chainReady = false;
+        return CSG_OK;
+    }
+    if (M < 0 || !chain) return fail(c, CSG_ERR_ARGUMENT, "need a chain of length at least 1");
     HIPCHECK(c, hipSetDevice(c->device));
     dropGraph(S);
     const double ke[2] = {kineticEnergy, 1.0};

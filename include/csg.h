@@ -144,7 +144,8 @@ csg_status csg_system_get(csg_ctx* ctx, int32_t* face, double* bary, double* vel
  * chain can be read and set as well, for a resume that continues the same trajectory. csg_system_thermostat: M (0 when
  * no NVT step ran since csg_system_init), T, tau, chain (4 (M + 1) doubles: position, velocity, acceleration, mass of each
  * entry; nullable) and the kinetic energy the chain last computed; csg_system_set_thermostat takes the same values, and
- * the next csg_system_nvt with the same (T, tau, M) continues from them. */
+ * the next csg_system_nvt with the same (T, tau, M) continues from them (M = 0 clears the chain: the next NVT step
+ * initialises a new one). */
 csg_status csg_system_set_state(csg_ctx* ctx, const int32_t* face, const double* bary, const double* vel, const double* forces);
 csg_status csg_system_thermostat(csg_ctx* ctx, int32_t* M, double* T, double* tau, double* chain, double* kineticEnergy);
 csg_status csg_system_set_thermostat(csg_ctx* ctx, int32_t M, double T, double tau, const double* chain, double kineticEnergy);

@@ -52,3 +52,30 @@ def test_resume_continues_the_trajectory(tmp_path, integrator):
     finally:
         ca.close()
         cc.close()
+
+
+def test_chainless_checkpoint_clears_the_thermostat():
+    """A checkpoint taken before any NVT step, restored into a system whose chain already ran, leaves no chain: the
+    next NVT steps start a new one, as they do in the run the checkpoint came from."""
+    w = workloads.config(2, n=2000)
+    print("case checkpoint without chain", flush=True)
+    ca, cb = csa.Context(0), csa.Context(0)
+    try:
+        a = workloads.make_system(ca, w)
+        a.nve(w.dt, 2)
+        ck = a.checkpoint()
+        assert ck["nhParams"][0] == 0 and ck["nhChain"].shape == (0, 4)
+        a.nvt(w.dt, 0.5, 1.0, 2, 3)
+        fa, ba, va, _ = a.get()
+
+        b = workloads.make_system(cb, w)
+        b.nvt(w.dt, 0.5, 1.0, 2, 2)  # a chain that the restore must drop
+        b.restore(ck)
+        b.nvt(w.dt, 0.5, 1.0, 2, 3)
+        fb, bb, vb, _ = b.get()
+        om = OracleMesh(w.X, w.T)
+        np.testing.assert_allclose(om.positions(fb, bb), om.positions(fa, ba), atol=1e-10)
+        np.testing.assert_allclose(vb, va, atol=1e-9 * np.abs(va).max())
+    finally:
+        ca.close()
+        cb.close()
