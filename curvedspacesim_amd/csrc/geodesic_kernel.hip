@@ -231,13 +231,16 @@ __device__ inline void loadV(const DeviceMesh& M, int v, double* p) {
     p[2] = M.vx[3 * v + 2];
 }
 
-// start tangent (source-face frame) of the ray direction (rx, ry) of a window
+// start tangent (source-face frame) of the ray direction (rx, ry) of a window. Computed in FP32, the precision it is
+// stored in: the unit direction is within a few FP32 ulps of the FP64 one rounded.
 __device__ inline float2 startTangent(bool rooted, float2 rot, double rx, double ry) {
     if (!rooted) return rot;
-    double l = sqrt(rx * rx + ry * ry);
-    if (l == 0.0) return make_float2(0.f, 0.f);
-    double ux = rx / l, uy = ry / l;
-    return make_float2((float)(rot.x * ux - rot.y * uy), (float)(rot.y * ux + rot.x * uy));
+    const float fx = (float)rx, fy = (float)ry;
+    const float q = fx * fx + fy * fy;
+    if (q == 0.f) return make_float2(0.f, 0.f);
+    const float inv = rsqrtf(q);
+    const float ux = fx * inv, uy = fy * inv;
+    return make_float2(rot.x * ux - rot.y * uy, rot.y * ux + rot.x * uy);
 }
 
 struct VCand {  // one pending vertex candidate of a lane, carried from pass 1 to pass 2
@@ -274,6 +277,16 @@ __device__ inline void offerVertex(const TT& T, VCand& c, int slot, double d, fl
 
 // distance from the window's source (sx, sy) to the point b of its edge
 __device__ inline double endDist(double b, double sx, double sy) { return sqrt((b - sx) * (b - sx) + sy * sy); }
+
+// A lower bound of endDist from an FP32 square root: the FP32 rounding of the square and of its root stay far below
+// kLowRel, so endDist >= endDistLow. Used where a new window is tested for pruning only (a pruned window would
+// have been pruned by the exact test too; one kept by the bound and not by the exact test is pruned by the exact test
+// when it is taken from the queue, against vertex distances that can only have dropped since).
+constexpr double kLowRel = 1e-6;
+__device__ inline double endDistLow(double b, double sx, double sy) {
+    const double q = (b - sx) * (b - sx) + sy * sy;
+    return (double)sqrtf((float)q) * (1.0 - kLowRel);
+}
 
 // lower bound of the distances the window carries: its source distance plus the distance to the nearest interval point
 // (r0, r1: endDist of b0 and b1)
@@ -354,14 +367,14 @@ template <class TT>
 __device__ void addWindow(const Ctx<TT>& C, int fs, int k, double L, double b0, double b1, double sx, double sy, double sg, bool rooted,
                           float2 rot, VCand& cA, VCand& cB) {
     if (!(b1 > b0)) return;
-    const double r0 = endDist(b0, sx, sy), r1 = endDist(b1, sx, sy);
+    const double r0 = endDistLow(b0, sx, sy), r1 = endDistLow(b1, sx, sy);  // lower bounds: pruning tests only
     if (keyOf(b0, b1, sx, sy, sg, r0, r1) > C.rMax) return;
     const TT& T = C.T;
     int vA = T.fcv[3 * fs + k], vB = T.fcv[3 * fs + (k + 1) % 3];
     double tol = kEdgeTol * L;
-    // an interval end on an edge end: the distance to that vertex (r0 or r1 when the end is exactly the vertex)
-    if (b0 <= tol) offerVertex(T, cA, vA, sg + (b0 == 0.0 ? r0 : endDist(0.0, sx, sy)), startTangent(rooted, rot, -sx, -sy));
-    if (b1 >= L - tol) offerVertex(T, cB, vB, sg + (b1 == L ? r1 : endDist(L, sx, sy)), startTangent(rooted, rot, L - sx, -sy));
+    // an interval end on an edge end: the distance to that vertex
+    if (b0 <= tol) offerVertex(T, cA, vA, sg + endDist(0.0, sx, sy), startTangent(rooted, rot, -sx, -sy));
+    if (b1 >= L - tol) offerVertex(T, cB, vB, sg + endDist(L, sx, sy), startTangent(rooted, rot, L - sx, -sy));
     if (filtered(T, vA, vB, L, b0, b1, sg, r0, r1)) return;
     pushItem(C, b0, b1, sx, sy, sg, fs * 8 + k * 2 + (rooted ? 1 : 0), rot);
 }
@@ -373,17 +386,17 @@ template <class TT>
 __device__ void addChild(const Ctx<TT>& C, int fs, int k, double L, double b0, double b1, double sx, double sy, double sg, bool rooted,
                          float2 rot, int vA, double& dA, int vB, double& dB, VCand& cA, VCand& cB) {
     if (!(b1 > b0)) return;
-    const double r0 = endDist(b0, sx, sy), r1 = endDist(b1, sx, sy);
+    const double r0 = endDistLow(b0, sx, sy), r1 = endDistLow(b1, sx, sy);  // lower bounds: pruning tests only
     if (keyOf(b0, b1, sx, sy, sg, r0, r1) > C.rMax) return;
     const TT& T = C.T;
     double tol = kEdgeTol * L;
     if (b0 <= tol) {
-        const double d = sg + (b0 == 0.0 ? r0 : endDist(0.0, sx, sy));
+        const double d = sg + endDist(0.0, sx, sy);
         offerVertex(T, cA, vA, d, startTangent(rooted, rot, -sx, -sy));
         dA = fmin(dA, d);
     }
     if (b1 >= L - tol) {
-        const double d = sg + (b1 == L ? r1 : endDist(L, sx, sy));
+        const double d = sg + endDist(L, sx, sy);
         offerVertex(T, cB, vB, d, startTangent(rooted, rot, L - sx, -sy));
         dB = fmin(dB, d);
     }
