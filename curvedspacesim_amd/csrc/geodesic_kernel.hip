@@ -275,8 +275,28 @@ __device__ inline void offerVertex(const TT& T, VCand& c, int slot, double d, fl
 }
 
 
+// FP64 quotient and square root for the propagation: the hardware reciprocal (square root) estimate, one Newton step and
+// a final correction (Markstein, IBM J. Res. Develop. 34 (1990) 111-119). For operands in the normal range they agree
+// with the correctly rounded results (no difference over 4M random operands of 10^-30..10^4 on gfx950); operands near
+// the bottom of the range, where the estimate loses accuracy, take the correctly rounded form. The general forms also
+// rescale every operand, which costs about half again as many instructions.
+__device__ inline double quot(double a, double b) {
+    if (fabs(b) < 1e-280) return a / b;
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(fma(-b, r, 1.0), r, r);
+    const double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+}
+__device__ inline double root(double v) {
+    if (!(v > 1e-280)) return sqrt(v);  // also 0, a negative value or NaN
+    double y = __builtin_amdgcn_rsq(v);
+    y = fma(fma(-v * y, 0.5 * y, 0.5), y, y);
+    const double s = v * y;
+    return fma(fma(-s, s, v), 0.5 * y, s);
+}
+
 // distance from the window's source (sx, sy) to the point b of its edge
-__device__ inline double endDist(double b, double sx, double sy) { return sqrt((b - sx) * (b - sx) + sy * sy); }
+__device__ inline double endDist(double b, double sx, double sy) { return root((b - sx) * (b - sx) + sy * sy); }
 
 // A lower bound of endDist from an FP32 square root: the FP32 rounding of the square and of its root stay far below
 // kLowRel, so endDist >= endDistLow. Used where a new window is tested for pruning only (a pruned window would
@@ -440,7 +460,7 @@ __device__ void windowTargets(const Ctx<TT>& C, int fs, int g, int k, double L, 
         double qx = bk1 * L + bk2 * x2, qy = bk2 * y2;
         double rx = qx - sx, ry = qy - sy;
         double ax = b0 - sx, ay = -sy, bx = b1 - sx, by = -sy;
-        double rl = sqrt(rx * rx + ry * ry);
+        double rl = root(rx * rx + ry * ry);
         double c1 = bx * ry - by * rx;  // cross(interval end ray, target ray)
         double c2 = rx * ay - ry * ax;  // cross(target ray, interval start ray)
         double tol = 1e-12 * rl;
@@ -483,10 +503,10 @@ __device__ void propagate(const Ctx<TT>& C, int fk, int g, double L, double x2, 
     int fs = fk >> 3, k = (fk >> 1) & 3;
     bool rooted = fk & 1;
     windowTargets<false, TT>(C, fs, g, k, L, x2, y2, b0, b1, sx, sy, sg, r0, r1, rooted, rot);
-    double xs = sx + (x2 - sx) * (-sy) / (y2 - sy);  // where the ray through the opposite corner crosses the entry edge
+    double xs = sx + quot((x2 - sx) * (-sy), y2 - sy);  // where the ray through the opposite corner crosses the entry edge
     double tolE = kEdgeTol * L;
     if (xs >= b0 - tolE && xs <= b1 + tolE) {
-        const double d = sg + sqrt((x2 - sx) * (x2 - sx) + (y2 - sy) * (y2 - sy));
+        const double d = sg + root((x2 - sx) * (x2 - sx) + (y2 - sy) * (y2 - sy));
         offerVertex(T, cP2, K.vC, d, startTangent(rooted, rot, x2 - sx, y2 - sy));
         K.dC = fmin(K.dC, d);
     }
@@ -499,7 +519,7 @@ __device__ void propagate(const Ctx<TT>& C, int fk, int g, double L, double x2, 
                 double dx = bx - sx, dy = -sy;
                 double dpx = dx * ux + dy * uy, dpy = ux * dy - uy * dx;
                 if (dpy <= 0) return dpx < 0 ? 0.0 : lL;
-                return fmin(lL, fmax(0.0, spx + dpx * (-spy) / dpy));
+                return fmin(lL, fmax(0.0, spx + quot(dpx * (-spy), dpy)));
             };
             double t0 = (b0 <= 0) ? 0.0 : hit(b0);
             double t1 = (xs <= b1) ? lL : hit(b1);
@@ -517,7 +537,7 @@ __device__ void propagate(const Ctx<TT>& C, int fk, int g, double L, double x2, 
                 double dx = bx - sx, dy = -sy;
                 double dpx = dx * ux + dy * uy, dpy = ux * dy - uy * dx;
                 if (dpy <= 0) return dpx < 0 ? 0.0 : lR;
-                return fmin(lR, fmax(0.0, spx + dpx * (-spy) / dpy));
+                return fmin(lR, fmax(0.0, spx + quot(dpx * (-spy), dpy)));
             };
             double t0 = (xs >= b0) ? 0.0 : hit(b0);
             double t1 = (b1 >= L) ? lR : hit(b1);
