@@ -299,9 +299,9 @@ __device__ inline double root(double v) {
 __device__ inline double endDist(double b, double sx, double sy) { return root((b - sx) * (b - sx) + sy * sy); }
 
 // A lower bound of endDist from an FP32 square root: the FP32 rounding of the square and of its root stay far below
-// kLowRel, so endDist >= endDistLow. Used where a new window is tested for pruning only (a pruned window would
-// have been pruned by the exact test too; one kept by the bound and not by the exact test is pruned by the exact test
-// when it is taken from the queue, against vertex distances that can only have dropped since).
+// kLowRel, so endDist >= endDistLow. The pruning tests (rMax and the edge-end filter) use it, so they can only keep
+// windows that exact tests would drop, never drop one they would keep; such a window is useless, and propagating it
+// changes no distance.
 constexpr double kLowRel = 1e-6;
 __device__ inline double endDistLow(double b, double sx, double sy) {
     const double q = (b - sx) * (b - sx) + sy * sy;
@@ -309,7 +309,7 @@ __device__ inline double endDistLow(double b, double sx, double sy) {
 }
 
 // lower bound of the distances the window carries: its source distance plus the distance to the nearest interval point
-// (r0, r1: endDist of b0 and b1)
+// (r0, r1: endDist of b0 and b1, or lower bounds of them)
 __device__ inline double keyOf(double b0, double b1, double sx, double sy, double sg, double r0, double r1) {
     const double dm = sx < b0 ? r0 : (sx > b1 ? r1 : -sy);
     return sg + dm;
@@ -985,7 +985,10 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
         bool live = false;
         if (mine) {
             if (kind == 0) {
-                const double r0 = endDist(b0, sx, sy), r1 = endDist(b1, sx, sy);
+                // lower bounds of the interval-end distances: a window near the pruning thresholds is propagated
+                // rather than dropped (propagating a useless window changes no distance); the same values scale the
+                // target tests' tolerance in both passes
+                const double r0 = endDistLow(b0, sx, sy), r1 = endDistLow(b1, sx, sy);
                 const int k1 = k == 2 ? 0 : k + 1, k2 = k == 0 ? 2 : k - 1;
                 Corners K;
                 K.vA = T.fcv[3 * fs + k];
@@ -1014,7 +1017,7 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
         ldsBarrier();
         if (live) {
             if (kind == 0 && T.fhead[fs] >= 0)
-                windowTargets<true, TT>(C, fs, g, k, L, x2, y2, b0, b1, sx, sy, sg, endDist(b0, sx, sy), endDist(b1, sx, sy), fk & 1,
+                windowTargets<true, TT>(C, fs, g, k, L, x2, y2, b0, b1, sx, sy, sg, endDistLow(b0, sx, sy), endDistLow(b1, sx, sy), fk & 1,
                                         rot);
             finishVertexCand(C, cP0);
             finishVertexCand(C, cP1);
