@@ -562,12 +562,23 @@ __device__ void fanOutVertex(const Ctx<TT>& C, int vslot, double d) {
     }
 }
 
+// Pass 2 for a lane's three vertex candidates (distinct corners of one face): a candidate whose offer still stands
+// sets the vertex's tangent and, for a pseudo-source vertex, pushes its event. The six LDS reads are issued together
+// (slots clamped to 0 for empty candidates), so they cost one round trip.
 template <class TT>
-__device__ inline void finishVertexCand(const Ctx<TT>& C, const VCand& c) {
-    if (c.slot < 0 || !c.improved) return;
-    if (C.T.vdist[c.slot] != dbits(c.d)) return;
-    C.T.vtan[c.slot] = c.t;
-    if (C.T.vflg[c.slot] && c.d <= C.rMax) pushVertexEvent(C, c.slot, c.d);
+__device__ inline void finishVertexCands(const Ctx<TT>& C, const VCand& c0, const VCand& c1, const VCand& c2) {
+    const TT& T = C.T;
+    const int s0 = c0.slot < 0 ? 0 : c0.slot, s1 = c1.slot < 0 ? 0 : c1.slot, s2 = c2.slot < 0 ? 0 : c2.slot;
+    const unsigned long long d0 = T.vdist[s0], d1 = T.vdist[s1], d2 = T.vdist[s2];
+    const uint8_t f0 = T.vflg[s0], f1 = T.vflg[s1], f2 = T.vflg[s2];
+    auto finish = [&](const VCand& c, unsigned long long d, uint8_t f) {
+        if (c.slot < 0 || !c.improved || d != dbits(c.d)) return;
+        T.vtan[c.slot] = c.t;
+        if (f && c.d <= C.rMax) pushVertexEvent(C, c.slot, c.d);
+    };
+    finish(c0, d0, f0);
+    finish(c1, d1, f1);
+    finish(c2, d2, f2);
 }
 
 __device__ inline double wsum(double x) {
@@ -1019,9 +1030,7 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
             if (kind == 0 && T.fhead[fs] >= 0)
                 windowTargets<true, TT>(C, fs, g, k, L, x2, y2, b0, b1, sx, sy, sg, endDistLow(b0, sx, sy), endDistLow(b1, sx, sy), fk & 1,
                                         rot);
-            finishVertexCand(C, cP0);
-            finishVertexCand(C, cP1);
-            finishVertexCand(C, cP2);
+            finishVertexCands(C, cP0, cP1, cP2);
         }
         __syncthreads();  // a full barrier: this step's spill stores are visible to the next batches
         h += nRing;
