@@ -263,6 +263,18 @@ __device__ inline void offerVertexTail(VCand& c, int slot, double d, float2 t, b
     c.t = t;
 }
 
+// Offer to a vertex whose distance the lane has read (at the start of the step, lowered since by its own offers), made
+// only when d is below that value: the atomic minimum's old value is not awaited, and the offer counts as an
+// improvement. If another lane offers the identical distance in the same step, both lanes then take it for theirs
+// and push the vertex's event twice, which costs a duplicate fan-out and changes no distance.
+template <class TT>
+__device__ inline void offerVertexKnown(const TT& T, VCand& c, int slot, double d, float2 t) {
+    if (slot < 0) return;
+    if (c.slot == slot && !(d < c.d)) return;
+    atomicMin(&T.vdist[slot], dbits(d));
+    offerVertexTail(c, slot, d, t, true);
+}
+
 // Offers distance d (start tangent t) to a vertex slot: atomic minimum, remembered in the lane's candidate c for the
 // second pass.
 template <class TT>
@@ -412,13 +424,17 @@ __device__ void addChild(const Ctx<TT>& C, int fs, int k, double L, double b0, d
     double tol = kEdgeTol * L;
     if (b0 <= tol) {
         const double d = sg + endDist(0.0, sx, sy);
-        offerVertex(T, cA, vA, d, startTangent(rooted, rot, -sx, -sy));
-        dA = fmin(dA, d);
+        if (d < dA) {
+            offerVertexKnown(T, cA, vA, d, startTangent(rooted, rot, -sx, -sy));
+            dA = d;
+        }
     }
     if (b1 >= L - tol) {
         const double d = sg + endDist(L, sx, sy);
-        offerVertex(T, cB, vB, d, startTangent(rooted, rot, L - sx, -sy));
-        dB = fmin(dB, d);
+        if (d < dB) {
+            offerVertexKnown(T, cB, vB, d, startTangent(rooted, rot, L - sx, -sy));
+            dB = d;
+        }
     }
     if (filteredBy(dA, dB, L, b0, b1, sg, r0, r1)) return;
     pushItem(C, b0, b1, sx, sy, sg, fs * 8 + k * 2 + (rooted ? 1 : 0), rot);
@@ -507,8 +523,10 @@ __device__ void propagate(const Ctx<TT>& C, int fk, int g, double L, double x2, 
     double tolE = kEdgeTol * L;
     if (xs >= b0 - tolE && xs <= b1 + tolE) {
         const double d = sg + root((x2 - sx) * (x2 - sx) + (y2 - sy) * (y2 - sy));
-        offerVertex(T, cP2, K.vC, d, startTangent(rooted, rot, x2 - sx, y2 - sy));
-        K.dC = fmin(K.dC, d);
+        if (d < K.dC) {
+            offerVertexKnown(T, cP2, K.vC, d, startTangent(rooted, rot, x2 - sx, y2 - sy));
+            K.dC = d;
+        }
     }
     if (xs > b0) {  // left child: edge corner k -> corner k+2, entered from the neighbour across local edge k+2
         int nb = K.nbL;
