@@ -965,6 +965,7 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
         double b0 = 0, b1 = 0, sx = 0, sy = 0, sg = 0, L = 0, x2 = 0, y2 = 0;
         double cf[6] = {0, 0, 0, 0, 0, 0};
         float2 rot = make_float2(0.f, 0.f);
+        Corners K;
         if (mine) {
             if (lane < nRing) {
                 const int q = (h + lane) % T.qcap;
@@ -1006,6 +1007,19 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
                 if (kind == 0)
                     for (int i = 0; i < 6; ++i) cf[i] = M.fchild[18 * g + 6 * k + i];
             }
+            if (kind == 0) {
+                // the face's corners and their distances as the previous step left them, read while the geometry
+                // loads are in flight (no distance changes before the barrier)
+                const int k1 = k == 2 ? 0 : k + 1, k2 = k == 0 ? 2 : k - 1;
+                K.vA = T.fcv[3 * fs + k];
+                K.vB = T.fcv[3 * fs + k1];
+                K.vC = T.fcv[3 * fs + k2];
+                K.nbL = T.fnb[3 * fs + k2];
+                K.nbR = T.fnb[3 * fs + k1];
+                K.dA = bitsd(T.vdist[K.vA]);
+                K.dB = bitsd(T.vdist[K.vB]);
+                K.dC = bitsd(T.vdist[K.vC]);
+            }
         }
         // once every lane holds its record, the ring slots [h, h + nRing) are free for this step's appends
         ldsBarrier();
@@ -1018,16 +1032,6 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
                 // rather than dropped (propagating a useless window changes no distance); the same values scale the
                 // target tests' tolerance in both passes
                 const double r0 = endDistLow(b0, sx, sy), r1 = endDistLow(b1, sx, sy);
-                const int k1 = k == 2 ? 0 : k + 1, k2 = k == 0 ? 2 : k - 1;
-                Corners K;
-                K.vA = T.fcv[3 * fs + k];
-                K.vB = T.fcv[3 * fs + k1];
-                K.vC = T.fcv[3 * fs + k2];
-                K.nbL = T.fnb[3 * fs + k2];
-                K.nbR = T.fnb[3 * fs + k1];
-                K.dA = bitsd(T.vdist[K.vA]);
-                K.dB = bitsd(T.vdist[K.vB]);
-                K.dC = bitsd(T.vdist[K.vC]);
                 live = keyOf(b0, b1, sx, sy, sg, r0, r1) <= C.rMax && !filteredBy(K.dA, K.dB, L, b0, b1, sg, r0, r1);
                 if (live) propagate(C, fk, g, L, x2, y2, b0, b1, sx, sy, sg, r0, r1, cf, rot, K, cP0, cP1, cP2);
             } else if (kind == 1) {
