@@ -715,9 +715,12 @@ __device__ void patchSearch(const GeoParams& P, const TT& T, Shared& sh, int* cu
                 }
             }
             __syncthreads();
+            // the new level's list becomes the frontier: the two lists swap roles (the same on every lane)
             const int nn = min(sh.nNext, frontCap);
-            for (int i = lane; i < nn; i += blockDim.x) cur[i] = nxt[i];
-            __syncthreads();
+            int* const done = cur;
+            cur = nxt;
+            nxt = done;
+            __syncthreads();  // every lane has read nNext before it is reset
             if (lane == 0) {
                 sh.nCur = sh.overflow ? 0 : nn;
                 sh.nNext = 0;
