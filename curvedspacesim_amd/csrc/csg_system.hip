@@ -38,6 +38,7 @@ struct SystemState {
     DevBuf cnt, off, toff, cand, maxd, rad;
     DevBuf disp, tstatus, gstatus, partial, counters, ke, chain, pstat, plist, pcount, spill, grab, redo, tspill;
     DevBuf scratch;
+    DevBuf scanTmp;  // tile sums and offsets of the tiled scan
     // learned screen: features and logits of the candidate pairs, kept-pair lists; recorded pair distances
     DevBuf sfeat, slogit, scnt, soff, scand, sparams, pdist;
     // independent particle groups: group of each particle; per group potential, squared range, patch-radius cap,
@@ -185,6 +186,68 @@ __global__ __launch_bounds__(1024) void kScanExclusive(const int* in, int* out, 
         run += v;
     }
     if (threadIdx.x == 0) out[n] = wtot[15];
+}
+
+// Tiled exclusive scan for long inputs (cell counts of large boxes, candidate counts of large systems): the tiles'
+// sums, the single-workgroup scan above over those sums, then each tile's own scan from its offset.
+constexpr int kScanTile = 4096;  // elements per 256-lane workgroup, a run of 16 per lane
+static long scanTiles(long n) { return (n + kScanTile - 1) / kScanTile; }
+
+__global__ __launch_bounds__(256) void kTileSums(const int* in, int* sums, int n) {
+    __shared__ int wsum[4];
+    const long base = (long)blockIdx.x * kScanTile;
+    int s = 0;
+    for (int k = threadIdx.x; k < kScanTile; k += 256) {
+        const long i = base + k;
+        if (i < n) s += in[i];
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) sums[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// tileOff[t]: the sum of the tiles before tile t; tileOff[nTiles]: the total, stored at out[n]
+__global__ __launch_bounds__(256) void kTileScan(const int* in, int* out, int n, const int* tileOff, int nTiles) {
+    __shared__ int wtot[4];
+    const long b = (long)blockIdx.x * kScanTile + 16 * (long)threadIdx.x;
+    int v[16];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        v[k] = b + k < n ? in[b + k] : 0;
+        s += v[k];
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = s;  // inclusive scan of the lanes' sums within the wavefront
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wtot[w] = x;
+    __syncthreads();
+    int run = tileOff[blockIdx.x] + x - s;
+    for (int q = 0; q < w; ++q) run += wtot[q];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        if (b + k < n) out[b + k] = run;
+        run += v[k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = tileOff[nTiles];
+}
+
+// exclusive scan of in[0, n) into out[0, n), with the total at out[n]; tmp holds 2 * (scanTiles(n) + 1) ints
+static void scanExclusive(hipStream_t st, const int* in, int* out, int n, int* tmp) {
+    const int tiles = (int)scanTiles(n);
+    if (tiles <= 1) {
+        hipLaunchKernelGGL(kScanExclusive, dim3(1), dim3(1024), 0, st, in, out, n);
+        return;
+    }
+    int* sums = tmp;
+    int* offs = tmp + tiles + 1;
+    hipLaunchKernelGGL(kTileSums, dim3(tiles), dim3(256), 0, st, in, sums, n);
+    hipLaunchKernelGGL(kScanExclusive, dim3(1), dim3(1024), 0, st, (const int*)sums, offs, tiles);
+    hipLaunchKernelGGL(kTileScan, dim3(tiles), dim3(256), 0, st, in, out, n, (const int*)offs, tiles);
 }
 
 __global__ void kScatter(int N, const int* key, const int* cellStart, int* cellFill, int* order) {
@@ -479,7 +542,7 @@ static csg_status sysNeighbors(csg_ctx* c, SystemState* S) {
     hipLaunchKernelGGL(kFillInt, dim3(blocksFor(S->nCells)), dim3(kBlock), 0, st, S->cellFill.as<int>(), 0, (long)S->nCells);
     hipLaunchKernelGGL(kCellKeys, dim3(blocksFor(N)), dim3(kBlock), 0, st, g, N, S->xyz.as<double>(), S->key.as<int>(),
                        S->cellCount.as<int>());
-    hipLaunchKernelGGL(kScanExclusive, dim3(1), dim3(1024), 0, st, S->cellCount.as<int>(), S->cellStart.as<int>(), S->nCells);
+    scanExclusive(st, S->cellCount.as<int>(), S->cellStart.as<int>(), S->nCells, S->scanTmp.as<int>());
     hipLaunchKernelGGL(kScatter, dim3(blocksFor(N)), dim3(kBlock), 0, st, N, S->key.as<int>(), S->cellStart.as<int>(), S->cellFill.as<int>(),
                        S->order.as<int>());
     hipLaunchKernelGGL(kSortCells, dim3(blocksFor(64L * S->nCells)), dim3(kBlock), 0, st, S->nCells, S->cellStart.as<int>(), S->order.as<int>());
@@ -492,7 +555,7 @@ static csg_status sysNeighbors(csg_ctx* c, SystemState* S) {
     hipLaunchKernelGGL((kCandidatesWave<false>), dim3(blocksFor(64L * N)), dim3(kBlock), 0, st, g, N, S->sxyz.as<double>(), S->key.as<int>(),
                        S->cellStart.as<int>(), S->order.as<int>(), r2, S->cnt.as<int>(), (const int*)nullptr, S->candCap, (int*)nullptr,
                        (double*)nullptr, (double*)nullptr, c->maximumDistance, S->counters.as<int>(), grp, gr2, gmaxd);
-    hipLaunchKernelGGL(kScanExclusive, dim3(1), dim3(1024), 0, st, S->cnt.as<int>(), S->off.as<int>(), N);
+    scanExclusive(st, S->cnt.as<int>(), S->off.as<int>(), N, S->scanTmp.as<int>());
     hipLaunchKernelGGL((kCandidatesWave<true>), dim3(blocksFor(64L * N)), dim3(kBlock), 0, st, g, N, S->sxyz.as<double>(), S->key.as<int>(),
                        S->cellStart.as<int>(), S->order.as<int>(), r2, S->cnt.as<int>(), S->off.as<int>(), S->candCap, S->cand.as<int>(),
                        S->maxd.as<double>(), S->rad.as<double>(), c->maximumDistance, S->counters.as<int>(), grp, gr2, gmaxd);
@@ -529,7 +592,7 @@ static csg_status sysScreen(csg_ctx* c, SystemState* S, GeoParams& G) {
                                     S->toff.as<int32_t>() + N, S->slogit.as<float>(), st));
     hipLaunchKernelGGL(kScreenCount, dim3(blocksFor(N)), dim3(kBlock), 0, st, N, S->toff.as<int>(), S->sfeat.as<float>(), S->slogit.as<float>(),
                        S->screenThreshold, S->scnt.as<int>());
-    hipLaunchKernelGGL(kScanExclusive, dim3(1), dim3(1024), 0, st, S->scnt.as<int>(), S->soff.as<int>(), N);
+    scanExclusive(st, S->scnt.as<int>(), S->soff.as<int>(), N, S->scanTmp.as<int>());
     hipLaunchKernelGGL(kScreenScatter, dim3(blocksFor(N)), dim3(kBlock), 0, st, N, S->toff.as<int>(), S->cand.as<int>(), S->sfeat.as<float>(),
                        S->slogit.as<float>(), S->screenThreshold, S->soff.as<int>(), S->scand.as<int>(), S->counters.as<int>());
     HIPCHECK(c, hipGetLastError());
@@ -806,6 +869,7 @@ csg_status csg_system_configure(csg_ctx* c, const csg_potential* pot, double ran
         if (S->candCap >= (1L << 31)) return fail(c, CSG_ERR_CAPACITY, "all-to-all pair count exceeds 2^31");
     }
     HIPCHECK(c, S->cand.ensure(4 * (size_t)std::max<long>(S->candCap, 1)));
+    HIPCHECK(c, S->scanTmp.ensure(4 * 2 * ((size_t)scanTiles(std::max<long>(N, S->cellList ? S->nCells : 0)) + 1)));
     // HBM scratch for sources the LDS tables cannot hold, or for whole-mesh solves
     // sources that outgrow the LDS tables are rare (a few per force evaluation); many narrow workgroups take them
     long faces = c->submeshing ? std::max<int>(maxPatchFaces > 0 ? maxPatchFaces : 1536, 1024) : c->nF;
