@@ -32,7 +32,8 @@ def _tie_sources(osys):
     return np.array([np.any(fl[off[i]:off[i + 1]] & 2) for i in range(osys.N)])
 
 
-@pytest.mark.parametrize("cfg,n", [(2, 2000), (3, 4000), (1, 64), (3, 20000)])
+# N = 8192 fills exactly two 4,096-entry tiles of the candidate-offset scan; 20000 ends in a partial tile
+@pytest.mark.parametrize("cfg,n", [(2, 2000), (3, 4000), (1, 64), (3, 20000), (2, 8192)])
 def test_candidates_and_forces(ctx, cfg, n):
     w = workloads.config(cfg, n=n)
     print(f"case system forces config {cfg} N={n}", flush=True)
