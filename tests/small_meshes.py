@@ -100,3 +100,28 @@ def random_points(rng, nF, n):
     f = rng.integers(0, nF, n).astype(np.int32)
     b = rng.dirichlet([1.0, 1.0, 1.0], n)
     return f, b
+
+
+def cylinder(n_around, n_along, radius, h):
+    """Open prism-cylinder whose faces and corner orders are those of grid(n_around, n_along, chord, h) with the last
+    column of vertices identified with the first: the flat grid is its exact unrolling (chord = 2 R sin(pi/n)), so a
+    (face, barycentrics) pair names corresponding points on both. Outward normals."""
+    X = []
+    for j in range(n_along + 1):
+        for i in range(n_around):
+            t = 2 * np.pi * i / n_around
+            X.append([radius * np.cos(t), radius * np.sin(t), j * h])
+    X = np.array(X)
+    T = []
+    for j in range(n_along):
+        for i in range(n_around):
+            a = j * n_around + i
+            b = j * n_around + (i + 1) % n_around
+            c = (j + 1) * n_around + (i + 1) % n_around
+            d = (j + 1) * n_around + i
+            T += [[a, b, c], [a, c, d]]
+    return X, np.array(T)
+
+
+def cylinder_chord(n_around, radius):
+    return 2 * radius * np.sin(np.pi / n_around)

@@ -6,6 +6,7 @@ import pytest
 
 import curvedspacesim_amd as csa
 import small_meshes as sm
+import walk_cases
 import workloads as wl
 from oracle.pyoracle import OracleMesh
 
@@ -41,8 +42,12 @@ def test_walk_matches_oracle(ctx, rng, mesh, length):
     of, ob, od, ov, os_ = om.transport(f, b, disp, vecs)
     assert np.array_equal(gs, os_)
     ok = os_ >= 0
-    same_face = gf == of
-    assert same_face[ok].mean() > 0.999
+    # a walk that ends on an edge or vertex may be reported in either face that holds that point: every face
+    # mismatch is counted and must lie on a corner or edge of both faces (a barycentric coordinate at the clamp level)
+    diff = ok & (gf != of)
+    on_edge = (gb.min(1) < 1e-9) & (ob.min(1) < 1e-9)
+    print(f"face mismatches {int(diff.sum())} of {int(ok.sum())}, all on edges: {bool(np.all(on_edge[diff]))}")
+    assert np.all(on_edge[diff])
     P1 = wl.positions(X, T, gf[ok], gb[ok])
     P2 = wl.positions(X, T, of[ok], ob[ok])
     np.testing.assert_allclose(P1, P2, atol=1e-10)
@@ -66,6 +71,20 @@ def test_vertex_crossing_constructed(ctx):
     assert gf[0] == of[0]
     np.testing.assert_allclose(gb, ob, atol=1e-12)
     np.testing.assert_allclose(gv, ov, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", sorted(walk_cases.CASES))
+def test_vertex_crossing_worked_by_hand(ctx, case):
+    """The interior vertex crossings worked through the reference's steps by hand (tests/walk_cases.py)."""
+    c = walk_cases.CASES[case]()
+    ctx.upload_mesh(c["X"], c["T"])
+    print(f"case vertex crossing {case}", flush=True)
+    f2, b2, d2, v2, st = ctx.transport_batch([c["face"]], [c["bary"]], [c["disp"]], [[c["vec"]]])
+    assert st[0] >= 1
+    assert f2[0] == c["end_face"]
+    np.testing.assert_allclose(b2[0], c["end_bary"], atol=1e-9)
+    np.testing.assert_allclose(wl.positions(c["X"], c["T"], f2, b2)[0], c["end_point"], atol=1e-9)
+    np.testing.assert_allclose(v2[0, 0], c["end_vec"], atol=1e-9)
 
 
 def test_border_error_is_reported(ctx):

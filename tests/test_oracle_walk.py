@@ -1,8 +1,10 @@
 """Pins the oracle's displacement walk (restating triangulatedMeshSpace::transportParticleAndVectors) by exact
 geometry: flat meshes, cube unfoldings, speed preservation and reversibility."""
 import numpy as np
+import pytest
 
 import small_meshes as sm
+import walk_cases
 from oracle.pyoracle import OracleMesh
 
 
@@ -103,3 +105,17 @@ def test_reverse_walk_returns(rng):
     f3, b3, d3, v3, st3 = m.transport(f2, b2, -v2[:, 0] * L, None)
     P0, P3 = m.positions(f, b), m.positions(f3, b3)
     np.testing.assert_allclose(P3, P0, atol=1e-9)
+
+
+@pytest.mark.parametrize("case", sorted(walk_cases.CASES))
+def test_vertex_crossing_worked_by_hand(case):
+    """Interior vertex crossings worked through the reference's steps by hand (tests/walk_cases.py): the over-travel,
+    the stale corner copy, the vertex taken from the first edge hit and the heading and rotation rules."""
+    c = walk_cases.CASES[case]()
+    m = OracleMesh(c["X"], c["T"])
+    f2, b2, d2, v2, st = m.transport([c["face"]], [c["bary"]], [c["disp"]], [[c["vec"]]])
+    assert st[0] >= 1
+    assert f2[0] == c["end_face"]
+    np.testing.assert_allclose(b2[0], c["end_bary"], atol=1e-9)
+    np.testing.assert_allclose(m.positions(f2, b2)[0], c["end_point"], atol=1e-9)
+    np.testing.assert_allclose(v2[0, 0], c["end_vec"], atol=1e-9)
