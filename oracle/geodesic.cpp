@@ -16,6 +16,7 @@
 // Start tangents (at the source, pointing along the path) and end tangents (at the target, pointing away from the
 // source) follow the reference's convention (src/utility/meshUtilities.cpp:373-379).
 #include <algorithm>
+#include <memory>
 #include <cstring>
 #include <limits>
 #include <queue>
@@ -438,9 +439,9 @@ const char* orc_last_error(void) { return g_err.c_str(); }
 
 orc_mesh* orc_mesh_create(int nV, const double* X, int nF, const int* F) {
     try {
-        auto* om = new orc_mesh();
+        std::unique_ptr<orc_mesh> om(new orc_mesh());  // freed if the mesh checks throw
         buildMesh(om->m, nV, X, nF, F);
-        return om;
+        return om.release();
     } catch (std::exception& e) {
         g_err = e.what();
         return nullptr;
