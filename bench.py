@@ -97,16 +97,36 @@ def profiled_traffic(config):
     return None
 
 
-def screened_leg(ctx, w, sysm, steps, plain_steps_s):
-    """The learned screen: trained on the GPU on exact pairs of two other bumpy spheres (seeds 13378, 13379),
-    calibrated on two more (13380, 13381), then timed on this workload from its current state. The training systems live in
-    a context of their own, so that this workload's mesh and particles stay in place."""
+def profiled_counters(config):
+    """SQ counter ratios of the geodesic kernel from the committed profile of this workload (lines 'sq_<name>: value'),
+    or None."""
+    if config != 3:
+        return None
+    path = os.path.join(ROOT, "profiles", "geodesicForceLds_bumpy_sphere_L6_N20000_sq.md")
+    out = {}
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("sq_") and ":" in line:
+                    k, v = line.split(":", 1)
+                    out[k.strip()[3:]] = float(v)
+    except (OSError, ValueError):
+        return None
+    return out or None
+
+
+def screened_leg(ctx, w, sysm, steps, plain_steps_s, config):
+    """The learned screen: trained on the GPU on exact pairs of MD snapshots of two other systems of this configuration
+    (seeds 13378, 13379), calibrated on snapshots of two more (13380, 13381), then timed on this workload from its
+    current state. The training systems live in a context of their own, so that this workload's mesh and particles stay
+    in place."""
     import curvedspacesim_amd as csa
     from curvedspacesim_amd import screen, workloads
 
     t0 = time.perf_counter()
     tctx = csa.Context(0)
-    built = screen.build(tctx, config=3, train_seeds=(13378, 13379), holdout_seeds=(13380, 13381), steps=2000, batch=8192)
+    md = {} if config == 3 else dict(train_md_steps=(0, 20, 40), holdout_md_steps=(25, 50))
+    built = screen.build(tctx, config=config, train_seeds=(13378, 13379), holdout_seeds=(13380, 13381), steps=2000, batch=8192, **md)
     tctx.close()
     t_build = time.perf_counter() - t0
     log(f"screen built in {t_build:.1f} s: threshold {built['threshold']:.3f}, {built['train_pairs']} training pairs")
@@ -115,6 +135,7 @@ def screened_leg(ctx, w, sysm, steps, plain_steps_s):
     F0, E0 = sysm.compute_forces()
     d = sysm.pair_distances()
     X = sysm.screen_features()
+    rc = screen.pair_ranges(sysm, w)
     sysm.record_pairs(False)
     dropped = screen.dropped(ctx, built["params"], built["threshold"], X)
     sysm.set_screen(built["params"], built["threshold"])
@@ -138,19 +159,23 @@ def screened_leg(ctx, w, sysm, steps, plain_steps_s):
     sysm.set_screen(None)
     v = steps / el
     screen_ms = kt["screen"][0] / steps
-    # MLP forward: 2 (16 x 64 + 64 x 64 + 64) = 10,368 flop per candidate pair, on the BF16 matrix cores
-    mlp_tflops = st["candidates"] * 10368 / (screen_ms * 1e-3) / 1e12 if screen_ms > 0 else 0.0
+    # only the pairs of the guard band (separation at least 0.985 of the cutoff) are scored; the others are kept
+    # unscored. MLP forward: 2 (16 x 64 + 64 x 64 + 64) = 10,368 flop per scored pair, on the BF16 matrix cores
+    scored = int(np.sum(X[:, 0] <= screen.GUARD))
+    mlp_tflops = scored * 10368 / (screen_ms * 1e-3) / 1e12 if screen_ms > 0 else 0.0
     return {"value": v, "unit": "steps/s", "speedup_vs_plain": v / plain_steps_s, "pairs_solved_per_step": st["kept"],
             "candidate_pairs_per_step": st["candidates"], "dropped_fraction": 1 - st["kept"] / max(st["candidates"], 1),
             "forces_match_plain": match, "max_force_difference_rel": dF, "energy_difference_rel": dE,
-            "dropped_within_cutoff": int(np.sum(dropped & (d < w.range))),
-            "beyond_cutoff_fraction": float(np.mean(d >= w.range)), "threshold": built["threshold"],
+            "dropped_within_cutoff": int(np.sum(dropped & (d < rc))),
+            "beyond_cutoff_fraction": float(np.mean(d >= rc)), "threshold": built["threshold"],
             "training_pairs": built["train_pairs"],
             "holdout_beyond_cutoff_fraction": built["holdout_beyond"], "holdout_dropped_fraction": built["holdout_drop"],
             "train_and_calibrate_s": t_build, "kernel_ms_per_step": {k: val[0] / steps for k, val in kt.items()},
-            "screen_ms_per_step": screen_ms,
+            "screen_ms_per_step": screen_ms, "pairs_scored_per_step": scored,
             "mfma": {"achieved": mlp_tflops, "peak": 2500.0, "unit": "TFLOP/s", "frac": mlp_tflops / 2500.0,
-                     "note": "MLP forward flops over the whole screening launch group (features, forward, compaction)"}}
+                     "note": "MLP forward flops of the scored pairs over the whole screening launch group (candidate pass, "
+                             "band features and MLP, scan, compaction); the matrix-core rate of the MLP itself is that of "
+                             "the forward kernel over all candidates, in profiles/"}}
 
 
 def accuracy_check(w, face, bary, vel, threads):
@@ -272,7 +297,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "fp64 distances/forces, fp32 tangents",
         "data": "synthetic",
         "config": {"workload": w.name, "N": int(len(w.face)), "faces": int(len(w.T)), "cutoff": w.range,
                    "integrator": w.integrator, "dt": w.dt, "screen": "plain (no learned screen)",
@@ -289,12 +314,15 @@ def main():
         "mean_work_items": ps["items"] / len(w.face),
         "mean_source_us": {k: ps["us_" + k] / len(w.face) for k in ("patch", "propagate", "epilogue")},
         "lds_fallbacks_per_step": {k: v / (args.steps + args.warmup) for k, v in sysm.fallback_counts().items()},
-        "roofline": {"kernel": "geodesicForceLds", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args.config),
-                     "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_launch},
+        # the kernel is bound by the latency of its dependent LDS and memory round trips (the SQ counters of the committed
+        # profile: waves waiting most of their cycles, VALU and FP64 issue far from peak); its HBM roofline is reported
+        "roofline": {"kernel": "geodesicForceLds", "bound": "latency", "peak_basis": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(args.config), "avg_launch_ms": avg_ms,
+                     "algorithmic_bytes_per_launch": bytes_launch, "counters": profiled_counters(args.config)},
     }
-    if not args.no_screen and args.config == 3:
-        out["screened"] = screened_leg(ctx, w, sysm, args.steps, steps_s)
+    if not args.no_screen and args.config in (3, 5):
+        out["screened"] = screened_leg(ctx, w, sysm, args.steps, steps_s, args.config)
     if not args.no_cpu_baseline and w.meta.get("groups") is not None:
         log("no CPU baseline for particle groups: the restatement runs one system at a time")
         out["cpu_baseline"] = None

@@ -3,8 +3,10 @@ displacement walk as hand-written HIP kernels behind a C-ABI (include/csg.h), wi
 reference's simulation/model/force/updater classes (include/csg_host.h, curvedspacesim_amd/csrc/host/)."""
 from ._native import (CsgError, Context, System, bumpy_sphere, icosphere, jittered_torus, read_off, HARMONIC, GAUSSIAN,
                       TRUNC_GAUSSIAN, NO_POTENTIAL, PAIR_UNREACHED, PAIR_BEYOND, PAIR_SENTINEL, SCREEN_FEATURES,
-                      SCREEN_PARAMS, LDS_VARIANTS, LDS_VARIANT_NAMES, save_checkpoint, load_checkpoint)
+                      SCREEN_PARAMS, LDS_VARIANTS, LDS_VARIANT_NAMES, save_checkpoint, load_checkpoint, ERR_ARGUMENT, ERR_DEVICE,
+                      ERR_MESH, ERR_STATE, ERR_WALK, ERR_CAPACITY)
 
 __all__ = ["CsgError", "Context", "System", "bumpy_sphere", "icosphere", "jittered_torus", "read_off", "HARMONIC", "GAUSSIAN",
            "TRUNC_GAUSSIAN", "NO_POTENTIAL", "PAIR_UNREACHED", "PAIR_BEYOND", "PAIR_SENTINEL", "SCREEN_FEATURES",
-           "SCREEN_PARAMS", "LDS_VARIANTS", "LDS_VARIANT_NAMES", "save_checkpoint", "load_checkpoint"]
+           "SCREEN_PARAMS", "LDS_VARIANTS", "LDS_VARIANT_NAMES", "save_checkpoint", "load_checkpoint", "ERR_ARGUMENT",
+           "ERR_DEVICE", "ERR_MESH", "ERR_STATE", "ERR_WALK", "ERR_CAPACITY"]

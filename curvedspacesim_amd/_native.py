@@ -20,7 +20,11 @@ VP = C.c_void_p
 
 
 class CsgError(RuntimeError):
-    pass
+    """A C-ABI call failed; `code` is its csg_status (CSG_ERR_* of include/csg.h)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 class csg_potential(C.Structure):
@@ -28,6 +32,8 @@ class csg_potential(C.Structure):
 
 
 HARMONIC, GAUSSIAN, TRUNC_GAUSSIAN, NO_POTENTIAL = 0, 1, 2, 3
+# csg_status codes (include/csg.h)
+ERR_ARGUMENT, ERR_DEVICE, ERR_MESH, ERR_STATE, ERR_WALK, ERR_CAPACITY = -1, -2, -3, -4, -5, -6
 PAIR_UNREACHED, PAIR_BEYOND, PAIR_SENTINEL = 1, 4, 16
 SCREEN_FEATURES, SCREEN_PARAMS = 16, 5313
 # variants of the LDS geodesic kernel (CSG_LDS_VARIANTS), in the library's order
@@ -200,7 +206,7 @@ class Context:
 
     def _check(self, rc, what):
         if rc != 0:
-            raise CsgError(f"{what} failed ({rc}): {self.L.csg_last_error(self.h).decode()}")
+            raise CsgError(f"{what} failed ({rc}): {self.L.csg_last_error(self.h).decode()}", rc)
 
     @property
     def device_name(self):
@@ -487,13 +493,27 @@ class System:
 CHECKPOINT_FIELDS = ("time", "R3position", "barycentricPosition", "faceIndex", "velocity", "force", "nhChain", "nhParams")
 
 
+def _is_hdf5(path):
+    return str(path).endswith((".h5", ".hdf5"))
+
+
 def save_checkpoint(path, ckpt):
-    """Writes a System.checkpoint() to an uncompressed .npz archive of plain arrays."""
+    """Writes a System.checkpoint(): to an HDF5 file in the reference's simpleModelDatabase layout when the name ends
+    in .h5 or .hdf5 (curvedspacesim_amd.trajectory), else to an uncompressed .npz archive of plain arrays."""
+    if _is_hdf5(path):
+        from . import trajectory
+
+        return trajectory.save_checkpoint(path, ckpt)
     np.savez(path, **{k: np.asarray(ckpt[k]) for k in CHECKPOINT_FIELDS})
 
 
 def load_checkpoint(path):
-    """Reads a checkpoint written by save_checkpoint (plain arrays only; object arrays are refused)."""
+    """Reads a checkpoint written by save_checkpoint (the last record of an HDF5 file; from an .npz, plain arrays only,
+    object arrays are refused)."""
+    if _is_hdf5(path):
+        from . import trajectory
+
+        return trajectory.load_checkpoint(path)
     with np.load(path, allow_pickle=False) as z:
         missing = [k for k in CHECKPOINT_FIELDS if k not in z.files]
         if missing:

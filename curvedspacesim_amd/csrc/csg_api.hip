@@ -14,6 +14,52 @@
 #include "geodesic_kernel.h"
 #include "transport_kernel.h"
 
+namespace csg {
+
+long patchFaceBound(const csg_ctx* c, double radius) {
+    const long nF = c->nF;
+    const double cell = radius + c->maxEdge;
+    if (nF == 0 || !(cell > 0) || !std::isfinite(cell)) return nF;
+    long n[3];
+    double lo[3];
+    long double cells = 1;
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = c->lo[a];
+        n[a] = (long)std::floor((c->hi[a] - c->lo[a]) / cell) + 1;
+        cells *= n[a];
+    }
+    if (cells > 1e15L) return nF;  // a radius tiny against the mesh: no useful bound from bins
+    std::unordered_map<long long, int> count;
+    count.reserve(2 * (size_t)nF);
+    auto key = [&](long x, long y, long z) { return (long long)x + (long long)n[0] * ((long long)y + (long long)n[1] * z); };
+    std::vector<long long> keys(nF);
+    for (long f = 0; f < nF; ++f) {
+        long q[3];
+        for (int a = 0; a < 3; ++a)
+            q[a] = std::min(n[a] - 1, std::max(0L, (long)std::floor((c->faceCentroid[3 * f + a] - lo[a]) / cell)));
+        keys[f] = key(q[0], q[1], q[2]);
+        ++count[keys[f]];
+    }
+    long best = 0;
+    for (const auto& kv : count) {
+        const long long k = kv.first;
+        const long x = (long)(k % n[0]), y = (long)((k / n[0]) % n[1]), z = (long)(k / ((long long)n[0] * n[1]));
+        long sum = 0;
+        for (long dx = -1; dx <= 1; ++dx)
+            for (long dy = -1; dy <= 1; ++dy)
+                for (long dz = -1; dz <= 1; ++dz) {
+                    const long X = x + dx, Y = y + dy, Z = z + dz;
+                    if (X < 0 || Y < 0 || Z < 0 || X >= n[0] || Y >= n[1] || Z >= n[2]) continue;
+                    auto it = count.find(key(X, Y, Z));
+                    if (it != count.end()) sum += it->second;
+                }
+        best = std::max(best, sum);
+    }
+    return std::min(nF, best);
+}
+
+}  // namespace csg
+
 namespace {
 
 using csg::fail;
@@ -106,6 +152,8 @@ csg_status preprocessMesh(csg_ctx* c, int nV, const double* X, int nF, const int
         }
     std::vector<double> geo(9 * (size_t)nF), nrm(3 * (size_t)nF);
     std::vector<double> angle(nV, 0.0);
+    c->faceCentroid.assign(3 * (size_t)nF, 0.0f);  // for the bound on patch sizes (csg::patchFaceBound)
+    c->maxEdge = 0;
     std::vector<uint8_t> flag(nV, 0);
     auto P = [&](int f, int k) { return X + 3 * F[3 * f + ((k % 3) + 3) % 3]; };
     c->area = 0;
@@ -122,6 +170,8 @@ csg_status preprocessMesh(csg_ctx* c, int nV, const double* X, int nF, const int
             for (int a = 0; a < 3; ++a) n[a] /= nl;
             double ey[3] = {n[1] * ex[2] - n[2] * ex[1], n[2] * ex[0] - n[0] * ex[2], n[0] * ex[1] - n[1] * ex[0]};
             geo[9 * f + 3 * k] = L;
+            c->maxEdge = std::max(c->maxEdge, L);
+            for (int a = 0; a < 3; ++a) c->faceCentroid[3 * (size_t)f + a] += (float)(p0[a] / 3.0);
             geo[9 * f + 3 * k + 1] = d2[0] * ex[0] + d2[1] * ex[1] + d2[2] * ex[2];
             geo[9 * f + 3 * k + 2] = d2[0] * ey[0] + d2[1] * ey[1] + d2[2] * ey[2];
             if (k == 0) {

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <string>
+#include <vector>
 
 #include "../../include/csg.h"
 #include "csg_common.h"
@@ -52,6 +53,8 @@ struct csg_ctx {
     double area = 0, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     DevBuf vx, fv, fadj, fgeo, fnrm, vflag, vfoff, vflist, curv, fchild, fcorn, fnbv;
     csg::DeviceMesh dm{};
+    std::vector<float> faceCentroid;  // host copy, 3 per face
+    double maxEdge = 0;               // longest edge of the mesh
     bool haveMesh = false;
     int submeshing = 0;
     double maximumDistance = 0;
@@ -74,6 +77,11 @@ struct csg_ctx {
 };
 
 namespace csg {
+
+// An upper bound on the faces of any submesh patch of radius `radius` (a face joins a patch when a corner lies within
+// the radius of the source, so its centroid lies within radius + the longest edge): with the centroids binned in cubic
+// cells of that size, every patch lies in the 27 cells around its source's cell. Host work, linear in the faces.
+long patchFaceBound(const csg_ctx* c, double radius);
 
 inline csg_status fail(csg_ctx* c, csg_status s, const std::string& msg) {
     if (c) c->err = msg;

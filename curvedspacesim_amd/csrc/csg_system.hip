@@ -23,6 +23,7 @@
 #include "csg_common.h"
 #include "csg_context.h"
 #include "geodesic_kernel.h"
+#include "scan_math.h"
 #include "screen_kernel.h"
 #include "transport_kernel.h"
 
@@ -40,14 +41,15 @@ struct SystemState {
     DevBuf scratch;
     DevBuf scanTmp;  // tile sums and offsets of the tiled scan
     // learned screen: features and logits of the candidate pairs, kept-pair lists; recorded pair distances
-    DevBuf sfeat, slogit, scnt, soff, scand, sparams, pdist;
+    DevBuf sfeat, scnt, soff, scand, sparams, pdist, skeep;
     // independent particle groups: group of each particle; per group potential, squared range, patch-radius cap,
     // truncation radius and sentinel distance
     int nGroups = 0;
-    DevBuf grp, gpot, gr2, gmaxd, grmax, gsent;
+    DevBuf grp, gpot, gr2, gmaxd, grmax, gsent, grange;
     std::vector<Potential> hgpot;
     std::vector<double> hgrange, hgmaxd;
     bool screenOn = false, recordPairs = false;
+    int reported[3] = {0, 0, 0};  // error counters (candidate overflow, unsolved, walk failures) at the last check
     int ldsVariant = 0;  // workgroup size of the LDS geodesic kernel, chosen by csg_system_autotune
     // one MD step captured as a HIP graph (replayed per step), and what it was captured for
     bool useGraph = false;
@@ -56,6 +58,12 @@ struct SystemState {
     float screenThreshold = 0.f;
     ScratchLayout layout{};
     int scratchBlocks = 0;
+    // the last tier of the step: HBM tables that hold any patch of the configured radius (patchFaceBound), few
+    // workgroups, for the sources that outgrow every other table
+    DevBuf scratchBig;
+    ScratchLayout layoutBig{};
+    int bigBlocks = 0;
+    double bigRadius = 0;
     // configuration
     Potential pot{POT_NONE, 0, 0, 0};
     double range = 1.0, rMax = HUGE_VAL;
@@ -103,9 +111,9 @@ void destroySystem(SystemState* s) {
     if (s->graph) hipGraphExecDestroy(s->graph);
     DevBuf* all[] = {&s->face, &s->bary, &s->vel, &s->force, &s->mass, &s->energy, &s->xyz, &s->sxyz, &s->key, &s->cellCount, &s->cellStart,
                      &s->cellFill, &s->order, &s->cnt, &s->off, &s->toff, &s->cand, &s->maxd, &s->rad, &s->disp, &s->tstatus,
-                     &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->spill, &s->grab, &s->redo, &s->tspill, &s->scratch,
-                     &s->sfeat, &s->slogit, &s->scnt, &s->soff, &s->scand, &s->sparams, &s->pdist,
-                     &s->grp, &s->gpot, &s->gr2, &s->gmaxd, &s->grmax, &s->gsent};
+                     &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->spill, &s->grab, &s->redo, &s->tspill, &s->scratch, &s->scratchBig,
+                     &s->sfeat, &s->scnt, &s->soff, &s->scand, &s->sparams, &s->pdist, &s->skeep,
+                     &s->grp, &s->gpot, &s->gr2, &s->gmaxd, &s->grmax, &s->gsent, &s->grange};
     for (DevBuf* b : all) b->release();
     for (auto& m : s->marks) {
         hipEventDestroy(m.a);
@@ -153,39 +161,44 @@ __global__ void kCellKeys(Grid g, int N, const double* xyz, int* key, int* cellC
     atomicAdd(&cellCount[c], 1);
 }
 
-// exclusive prefix sum of n ints into out[0..n] (out[n] = total), one workgroup of 1024 lanes: each lane sums a
+// stores of a 64-bit running offset: saturated into int32 offset arrays, as is into the int64 tile offsets
+__device__ inline void storeOffset(int* p, long long v) { *p = saturateOffset(v); }
+__device__ inline void storeOffset(long long* p, long long v) { *p = v; }
+
+// exclusive prefix sum of n counts into out[0..n] (out[n] = total), one workgroup of 1024 lanes: each lane sums a
 // contiguous run of ceil(n / 1024) entries, the lanes' sums are scanned across each wavefront with shuffles and across
-// the 16 wavefronts through LDS (two barriers in all), and each lane writes its run
-__global__ __launch_bounds__(1024) void kScanExclusive(const int* in, int* out, int n) {
-    __shared__ int wtot[16];
+// the 16 wavefronts through LDS (two barriers in all), and each lane writes its run. Sums are 64-bit.
+template <class TIn, class TOut>
+__global__ __launch_bounds__(1024) void kScanExclusive(const TIn* in, TOut* out, int n) {
+    __shared__ long long wtot[16];
     const int per = (n + 1023) / 1024;
     const int b = min((int)threadIdx.x * per, n), e = min(b + per, n);
-    int s = 0;
+    long long s = 0;
     for (int i = b; i < e; ++i) s += in[i];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int x = s;  // inclusive scan of the lanes' sums within the wavefront
+    long long x = s;  // inclusive scan of the lanes' sums within the wavefront
     for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o, 64);
+        const long long y = __shfl_up(x, o, 64);
         if (lane >= o) x += y;
     }
     if (lane == 63) wtot[w] = x;
     __syncthreads();
     if (w == 0) {  // inclusive scan of the 16 wavefront totals
-        int t = lane < 16 ? wtot[lane] : 0;
+        long long t = lane < 16 ? wtot[lane] : 0;
         for (int o = 1; o < 16; o <<= 1) {
-            const int y = __shfl_up(t, o, 64);
+            const long long y = __shfl_up(t, o, 64);
             if (lane >= o) t += y;
         }
         if (lane < 16) wtot[lane] = t;
     }
     __syncthreads();
-    int run = x - s + (w > 0 ? wtot[w - 1] : 0);
+    long long run = x - s + (w > 0 ? wtot[w - 1] : 0);
     for (int i = b; i < e; ++i) {
-        const int v = in[i];
-        out[i] = run;
+        const long long v = in[i];
+        storeOffset(out + i, run);
         run += v;
     }
-    if (threadIdx.x == 0) out[n] = wtot[15];
+    if (threadIdx.x == 0) storeOffset(out + n, wtot[15]);
 }
 
 // Tiled exclusive scan for long inputs (cell counts of large boxes, candidate counts of large systems): the tiles'
@@ -193,10 +206,10 @@ __global__ __launch_bounds__(1024) void kScanExclusive(const int* in, int* out, 
 constexpr int kScanTile = 4096;  // elements per 256-lane workgroup, a run of 16 per lane
 static long scanTiles(long n) { return (n + kScanTile - 1) / kScanTile; }
 
-__global__ __launch_bounds__(256) void kTileSums(const int* in, int* sums, int n) {
-    __shared__ int wsum[4];
+__global__ __launch_bounds__(256) void kTileSums(const int* in, long long* sums, int n) {
+    __shared__ long long wsum[4];
     const long base = (long)blockIdx.x * kScanTile;
-    int s = 0;
+    long long s = 0;
     for (int k = threadIdx.x; k < kScanTile; k += 256) {
         const long i = base + k;
         if (i < n) s += in[i];
@@ -208,46 +221,47 @@ __global__ __launch_bounds__(256) void kTileSums(const int* in, int* sums, int n
 }
 
 // tileOff[t]: the sum of the tiles before tile t; tileOff[nTiles]: the total, stored at out[n]
-__global__ __launch_bounds__(256) void kTileScan(const int* in, int* out, int n, const int* tileOff, int nTiles) {
-    __shared__ int wtot[4];
+__global__ __launch_bounds__(256) void kTileScan(const int* in, int* out, int n, const long long* tileOff, int nTiles) {
+    __shared__ long long wtot[4];
     const long b = (long)blockIdx.x * kScanTile + 16 * (long)threadIdx.x;
     int v[16];
-    int s = 0;
+    long long s = 0;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         v[k] = b + k < n ? in[b + k] : 0;
         s += v[k];
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int x = s;  // inclusive scan of the lanes' sums within the wavefront
+    long long x = s;  // inclusive scan of the lanes' sums within the wavefront
     for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o, 64);
+        const long long y = __shfl_up(x, o, 64);
         if (lane >= o) x += y;
     }
     if (lane == 63) wtot[w] = x;
     __syncthreads();
-    int run = tileOff[blockIdx.x] + x - s;
+    long long run = tileOff[blockIdx.x] + x - s;
     for (int q = 0; q < w; ++q) run += wtot[q];
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-        if (b + k < n) out[b + k] = run;
+        if (b + k < n) out[b + k] = saturateOffset(run);
         run += v[k];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = tileOff[nTiles];
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = saturateOffset(tileOff[nTiles]);
 }
 
-// exclusive scan of in[0, n) into out[0, n), with the total at out[n]; tmp holds 2 * (scanTiles(n) + 1) ints
-static void scanExclusive(hipStream_t st, const int* in, int* out, int n, int* tmp) {
+// exclusive scan of in[0, n) into out[0, n), with the total at out[n] (saturating offsets); tmp holds
+// 2 * (scanTiles(n) + 1) 64-bit entries
+static void scanExclusive(hipStream_t st, const int* in, int* out, int n, long long* tmp) {
     const int tiles = (int)scanTiles(n);
     if (tiles <= 1) {
-        hipLaunchKernelGGL(kScanExclusive, dim3(1), dim3(1024), 0, st, in, out, n);
+        hipLaunchKernelGGL((kScanExclusive<int, int>), dim3(1), dim3(1024), 0, st, in, out, n);
         return;
     }
-    int* sums = tmp;
-    int* offs = tmp + tiles + 1;
+    long long* sums = tmp;
+    long long* offs = tmp + tiles + 1;
     hipLaunchKernelGGL(kTileSums, dim3(tiles), dim3(256), 0, st, in, sums, n);
-    hipLaunchKernelGGL(kScanExclusive, dim3(1), dim3(1024), 0, st, (const int*)sums, offs, tiles);
-    hipLaunchKernelGGL(kTileScan, dim3(tiles), dim3(256), 0, st, in, out, n, (const int*)offs, tiles);
+    hipLaunchKernelGGL((kScanExclusive<long long, long long>), dim3(1), dim3(1024), 0, st, (const long long*)sums, offs, tiles);
+    hipLaunchKernelGGL(kTileScan, dim3(tiles), dim3(256), 0, st, in, out, n, (const long long*)offs, tiles);
 }
 
 __global__ void kScatter(int N, const int* key, const int* cellStart, int* cellFill, int* order) {
@@ -327,7 +341,7 @@ __global__ void kCandidatesWave(Grid g, int N, const double* sxyz, const int* ke
                     const unsigned long long mask = __ballot(hit);
                     if (hit) {
                         const long pos = k + __popcll(mask & below);
-                        if (kFill && base + pos < cap) cand[base + pos] = order[q];
+                        if (kFill && candidateSlotValid(base, pos, cap)) cand[base + pos] = order[q];
                         if (d2 > m2) m2 = d2;
                     }
                     k += __popcll(mask);
@@ -338,7 +352,7 @@ __global__ void kCandidatesWave(Grid g, int N, const double* sxyz, const int* ke
     if (!kFill) {
         cnt[i] = (int)k;
     } else {
-        if (base + k > cap) atomicAdd(&counters[CNT_CAND_OVERFLOW], 1);
+        if (candidatesOverflow(base, k, cap)) atomicAdd(&counters[CNT_CAND_OVERFLOW], 1);
         const double md = sqrt(m2);
         maxd[i] = md;
         rad[i] = md < maxDist ? md : maxDist;
@@ -491,28 +505,13 @@ __global__ void kScaleVel(int N, double* v, const double* ke) {
 
 }  // namespace
 
-// the screen keeps a pair whose logit is at or above the threshold, and every pair closer than the guard distance
-__device__ inline bool screenKeeps(const float* feat, const float* logit, int q, float thr) {
-    return logit[q] >= thr || feat[(size_t)q * kScreenIn] > kScreenGuard;
-}
-
-// pairs of source i that the screen keeps
-__global__ void kScreenCount(int N, const int* toff, const float* feat, const float* logit, float thr, int* cnt) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    int c = 0;
-    for (int q = toff[i]; q < toff[i + 1]; ++q) c += screenKeeps(feat, logit, q, thr) ? 1 : 0;
-    cnt[i] = c;
-}
-
 // the kept candidates of each source, in their original order
-__global__ void kScreenScatter(int N, const int* toff, const int* cand, const float* feat, const float* logit, float thr, const int* soff,
-                               int* scand, int* counters) {
+__global__ void kScreenScatter(int N, const int* toff, const int* cand, const uint8_t* keep, const int* soff, int* scand, int* counters) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     int o = soff[i];
     for (int q = toff[i]; q < toff[i + 1]; ++q)
-        if (screenKeeps(feat, logit, q, thr)) scand[o++] = cand[q];
+        if (keep[q]) scand[o++] = cand[q];
     if (i == 0) counters[CNT_KEPT] = soff[N];
 }
 
@@ -542,7 +541,7 @@ static csg_status sysNeighbors(csg_ctx* c, SystemState* S) {
     hipLaunchKernelGGL(kFillInt, dim3(blocksFor(S->nCells)), dim3(kBlock), 0, st, S->cellFill.as<int>(), 0, (long)S->nCells);
     hipLaunchKernelGGL(kCellKeys, dim3(blocksFor(N)), dim3(kBlock), 0, st, g, N, S->xyz.as<double>(), S->key.as<int>(),
                        S->cellCount.as<int>());
-    scanExclusive(st, S->cellCount.as<int>(), S->cellStart.as<int>(), S->nCells, S->scanTmp.as<int>());
+    scanExclusive(st, S->cellCount.as<int>(), S->cellStart.as<int>(), S->nCells, S->scanTmp.as<long long>());
     hipLaunchKernelGGL(kScatter, dim3(blocksFor(N)), dim3(kBlock), 0, st, N, S->key.as<int>(), S->cellStart.as<int>(), S->cellFill.as<int>(),
                        S->order.as<int>());
     hipLaunchKernelGGL(kSortCells, dim3(blocksFor(64L * S->nCells)), dim3(kBlock), 0, st, S->nCells, S->cellStart.as<int>(), S->order.as<int>());
@@ -555,7 +554,7 @@ static csg_status sysNeighbors(csg_ctx* c, SystemState* S) {
     hipLaunchKernelGGL((kCandidatesWave<false>), dim3(blocksFor(64L * N)), dim3(kBlock), 0, st, g, N, S->sxyz.as<double>(), S->key.as<int>(),
                        S->cellStart.as<int>(), S->order.as<int>(), r2, S->cnt.as<int>(), (const int*)nullptr, S->candCap, (int*)nullptr,
                        (double*)nullptr, (double*)nullptr, c->maximumDistance, S->counters.as<int>(), grp, gr2, gmaxd);
-    scanExclusive(st, S->cnt.as<int>(), S->off.as<int>(), N, S->scanTmp.as<int>());
+    scanExclusive(st, S->cnt.as<int>(), S->off.as<int>(), N, S->scanTmp.as<long long>());
     hipLaunchKernelGGL((kCandidatesWave<true>), dim3(blocksFor(64L * N)), dim3(kBlock), 0, st, g, N, S->sxyz.as<double>(), S->key.as<int>(),
                        S->cellStart.as<int>(), S->order.as<int>(), r2, S->cnt.as<int>(), S->off.as<int>(), S->candCap, S->cand.as<int>(),
                        S->maxd.as<double>(), S->rad.as<double>(), c->maximumDistance, S->counters.as<int>(), grp, gr2, gmaxd);
@@ -565,8 +564,8 @@ static csg_status sysNeighbors(csg_ctx* c, SystemState* S) {
     return CSG_OK;
 }
 
-// features of every candidate pair of the last neighbour search (the pair count is read on the device)
-static void screenFeatures(csg_ctx* c, SystemState* S) {
+// feature inputs of the candidate pairs of the last neighbour search (the pair count is read on the device)
+static ScreenFeatureParams featureParams(csg_ctx* c, SystemState* S) {
     ScreenFeatureParams F{};
     F.mesh = c->dm;
     F.curv = c->curv.as<double>();
@@ -578,32 +577,37 @@ static void screenFeatures(csg_ctx* c, SystemState* S) {
     F.tidx = S->cand.as<int32_t>();
     F.patchCount = c->submeshing ? S->pcount.as<int32_t>() : nullptr;
     F.cutoff = S->range;
+    if (S->nGroups) {
+        F.srcGroup = S->grp.as<int32_t>();
+        F.groupRange = S->grange.as<double>();
+    }
     F.nPairsDev = S->toff.as<int32_t>() + S->N;
     F.out = S->sfeat.as<float>();
-    (void)launchScreenFeatures(F, (int)std::min<long>(S->candCap, 0x7fffffff), c->stream);
+    return F;
+}
+
+// features of every candidate pair of the last neighbour search
+static void screenFeatures(csg_ctx* c, SystemState* S) {
+    (void)launchScreenFeatures(featureParams(c, S), (int)std::min<long>(S->candCap, 0x7fffffff), c->stream);
 }
 
 // screen the candidate pairs and point the solver at the kept lists
 static csg_status sysScreen(csg_ctx* c, SystemState* S, GeoParams& G) {
     const int N = S->N;
     hipStream_t st = c->stream;
-    screenFeatures(c, S);
-    HIPCHECK(c, launchScreenForward(S->sparams.as<float>(), S->sfeat.as<float>(), (int)std::min<long>(S->candCap, 0x7fffffff),
-                                    S->toff.as<int32_t>() + N, S->slogit.as<float>(), st));
-    hipLaunchKernelGGL(kScreenCount, dim3(blocksFor(N)), dim3(kBlock), 0, st, N, S->toff.as<int>(), S->sfeat.as<float>(), S->slogit.as<float>(),
-                       S->screenThreshold, S->scnt.as<int>());
-    scanExclusive(st, S->scnt.as<int>(), S->soff.as<int>(), N, S->scanTmp.as<int>());
-    hipLaunchKernelGGL(kScreenScatter, dim3(blocksFor(N)), dim3(kBlock), 0, st, N, S->toff.as<int>(), S->cand.as<int>(), S->sfeat.as<float>(),
-                       S->slogit.as<float>(), S->screenThreshold, S->soff.as<int>(), S->scand.as<int>(), S->counters.as<int>());
+    HIPCHECK(c, launchScreenSources(featureParams(c, S), S->sparams.as<float>(), S->screenThreshold, S->skeep.as<uint8_t>(), S->scnt.as<int>(),
+                                    st));
+    scanExclusive(st, S->scnt.as<int>(), S->soff.as<int>(), N, S->scanTmp.as<long long>());
+    hipLaunchKernelGGL(kScreenScatter, dim3(blocksFor(N)), dim3(kBlock), 0, st, N, S->toff.as<int>(), S->cand.as<int>(), S->skeep.as<uint8_t>(),
+                       S->soff.as<int>(), S->scand.as<int>(), S->counters.as<int>());
     HIPCHECK(c, hipGetLastError());
     G.toff = S->soff.as<int32_t>();
     G.tidx = S->scand.as<int32_t>();
     return CSG_OK;
 }
 
-static csg_status sysForces(csg_ctx* c, SystemState* S) {
-    csg_status s = sysNeighbors(c, S);
-    if (s) return s;
+// parameters of the force computation of the current step (the kept lists when the screen is on)
+static GeoParams forceParams(csg_ctx* c, SystemState* S) {
     GeoParams G{};
     G.mesh = c->dm;
     G.pface = S->face.as<int32_t>();
@@ -639,6 +643,21 @@ static csg_status sysForces(csg_ctx* c, SystemState* S) {
         G.patchStride = kPatchListCap;
         G.spill = S->spill.as<SpillRecord>();
         G.spillCap = kSpillCap;
+    }
+    if (S->screenOn && c->submeshing) {
+        G.toff = S->soff.as<int32_t>();
+        G.tidx = S->scand.as<int32_t>();
+    }
+    return G;
+}
+
+static csg_status sysForces(csg_ctx* c, SystemState* S) {
+    csg_status s = sysNeighbors(c, S);
+    if (s) return s;
+    GeoParams G = forceParams(c, S);
+    G.toff = S->toff.as<int32_t>();  // the screen points the solver at its kept lists once they exist
+    G.tidx = S->cand.as<int32_t>();
+    if (c->submeshing) {
         {
             MarkScope mark(S, c->stream, MARK_PATCH);
             HIPCHECK(c, launchPatch(G, S->plist.as<int32_t>(), S->pcount.as<int32_t>(), kPatchListCap, c->stream));
@@ -664,6 +683,7 @@ static csg_status sysForces(csg_ctx* c, SystemState* S) {
                                                c->stream));
         }
         HIPCHECK(c, launchGeodesicScratchRedo(G, S->layout, S->scratchBlocks, c->stream));
+        if (S->bigBlocks) HIPCHECK(c, launchGeodesicScratchRedo(G, S->layoutBig, S->bigBlocks, c->stream));
     } else {
         MarkScope mark(S, c->stream, MARK_GEODESIC_HBM);
         HIPCHECK(c, launchGeodesicScratch(G, S->layout, nullptr, S->N, S->scratchBlocks, c->stream));
@@ -781,6 +801,99 @@ static csg_status runSteps(csg_ctx* c, SystemState* S, const double key[6], int 
     return CSG_OK;
 }
 
+// Sizes the step's last tier for patches of radius `radius` (the largest of the maximum distances and ranges in use):
+// tables for patchFaceBound faces and up to 32,768 targets, in as many workgroups (8 .. 256) as 2 GB hold.
+static csg_status sizeLastTier(csg_ctx* c, SystemState* S, double radius) {
+    if (!c->submeshing) {
+        S->bigBlocks = 0;
+        return CSG_OK;
+    }
+    const long faces = std::max<long>(patchFaceBound(c, radius), S->layout.fcap);
+    if (faces <= S->layout.fcap) {  // the fixed tier holds every patch already
+        S->bigBlocks = 0;
+        S->bigRadius = radius;
+        return CSG_OK;
+    }
+    ScratchLayout L = makeScratchLayout(faces, std::min<long>(std::max<long>(S->candCap, 1), 32768));
+    const int blocks = (int)std::max<long>(8, std::min<long>(256, (2L << 30) / (long)L.bytesPerBlock));
+    HIPCHECK(c, S->scratchBig.ensure(L.bytesPerBlock * (size_t)blocks));
+    L.base = S->scratchBig.p;
+    S->layoutBig = L;
+    S->bigBlocks = blocks;
+    S->bigRadius = radius;
+    return CSG_OK;
+}
+
+// Sources that every tier of the last force computation left unsolved (nonzero status: a patch, target list or
+// work-item volume beyond the step's fixed HBM tables), solved again with HBM tables that grow until each fits, as
+// csg_pair_forces does. Synchronous; the unsolved counter drops by the sources recovered.
+static csg_status resolveLeftovers(csg_ctx* c, SystemState* S) {
+    int cnt[CNT_N];
+    HIPCHECK(c, hipMemcpyAsync(cnt, S->counters.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (cnt[CNT_UNSOLVED] <= S->reported[CNT_UNSOLVED]) return CSG_OK;
+    const int N = S->N;
+    GeoParams G = forceParams(c, S);
+    std::vector<int> st(N), off(N + 1);
+    HIPCHECK(c, hipMemcpyAsync(st.data(), G.status, sizeof(int) * N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(off.data(), G.toff, sizeof(int) * (N + 1), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    std::vector<int> redo;
+    long maxT = 1;
+    for (int i = 0; i < N; ++i)
+        if (st[i]) {
+            redo.push_back(i);
+            maxT = std::max<long>(maxT, off[i + 1] - off[i]);
+        }
+    const int before = (int)redo.size();
+    long faces = c->submeshing ? std::max<long>(2L * S->layout.fcap, 4096) : c->nF;
+    for (int attempt = 0; attempt < 6 && !redo.empty(); ++attempt, faces *= 4) {
+        faces = std::min<long>(faces, c->nF);
+        const int blocks = std::min<int>((int)redo.size(), 512);
+        ScratchLayout L = makeScratchLayout(faces, maxT);
+        HIPCHECK(c, c->scratch.ensure(L.bytesPerBlock * (size_t)blocks));
+        L.base = c->scratch.p;
+        HIPCHECK(c, c->list.ensure(sizeof(int) * redo.size()));
+        HIPCHECK(c, hipMemcpyAsync(c->list.p, redo.data(), sizeof(int) * redo.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(c, launchGeodesicScratch(G, L, c->list.as<int32_t>(), (int)redo.size(), blocks, c->stream));
+        HIPCHECK(c, hipMemcpyAsync(st.data(), G.status, sizeof(int) * N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        std::vector<int> next;
+        for (int i : redo)
+            if (st[i]) next.push_back(i);
+        redo.swap(next);
+        if (faces >= c->nF) break;
+    }
+    const int left = cnt[CNT_UNSOLVED] - (before - (int)redo.size());
+    HIPCHECK(c, hipMemcpyAsync(S->counters.as<int>() + CNT_UNSOLVED, &left, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return CSG_OK;
+}
+
+// Reports what the error counters gained since the last check (csg_system_init starts them at zero): failed walks
+// as CSG_ERR_WALK (those particles keep their positions), candidate-capacity overflows and unsolved sources as
+// CSG_ERR_CAPACITY (an overflowed source lost candidates; an unsolved source got zero force). Waits for the stream.
+static csg_status checkCounters(csg_ctx* c, SystemState* S) {
+    int cnt[CNT_N];
+    HIPCHECK(c, hipMemcpyAsync(cnt, S->counters.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    int d[3];
+    for (int k = 0; k < 3; ++k) {
+        d[k] = cnt[k] - S->reported[k];
+        S->reported[k] = cnt[k];
+    }
+    if (d[CNT_WALK_FAIL] > 0)
+        return fail(c, CSG_ERR_WALK, std::to_string(d[CNT_WALK_FAIL]) + " particle walks failed since the last check (those particles kept their positions)");
+    if (d[CNT_CAND_OVERFLOW] > 0)
+        return fail(c, CSG_ERR_CAPACITY, std::to_string(d[CNT_CAND_OVERFLOW]) + " sources had more candidates than the candidate capacity since the last check");
+    if (d[CNT_UNSOLVED] > 0)
+        return fail(c, CSG_ERR_CAPACITY, std::to_string(d[CNT_UNSOLVED]) +
+                                             " sources exceeded the step's geodesic table capacity since the last check and got zero force "
+                                             "(raise maxPatchFaces of csg_system_configure, or compute forces with csg_system_compute_forces, which "
+                                             "grows its tables)");
+    return CSG_OK;
+}
+
 extern "C" {
 
 csg_status csg_system_init(csg_ctx* c, int32_t N, const int32_t* face, const double* bary, const double* vel, const double* mass) {
@@ -831,6 +944,7 @@ csg_status csg_system_init(csg_ctx* c, int32_t N, const int32_t* face, const dou
     HIPCHECK(c, S->ke.ensure(16));
     HIPCHECK(c, hipMemsetAsync(S->counters.p, 0, 4 * CNT_N, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
+    for (int& r : S->reported) r = 0;
     return CSG_OK;
 }
 
@@ -869,7 +983,7 @@ csg_status csg_system_configure(csg_ctx* c, const csg_potential* pot, double ran
         if (S->candCap >= (1L << 31)) return fail(c, CSG_ERR_CAPACITY, "all-to-all pair count exceeds 2^31");
     }
     HIPCHECK(c, S->cand.ensure(4 * (size_t)std::max<long>(S->candCap, 1)));
-    HIPCHECK(c, S->scanTmp.ensure(4 * 2 * ((size_t)scanTiles(std::max<long>(N, S->cellList ? S->nCells : 0)) + 1)));
+    HIPCHECK(c, S->scanTmp.ensure(8 * 2 * ((size_t)scanTiles(std::max<long>(N, S->cellList ? S->nCells : 0)) + 1)));
     // HBM scratch for sources the LDS tables cannot hold, or for whole-mesh solves
     // sources that outgrow the LDS tables are rare (a few per force evaluation); many narrow workgroups take them
     long faces = c->submeshing ? std::max<int>(maxPatchFaces > 0 ? maxPatchFaces : 1536, 1024) : c->nF;
@@ -880,6 +994,8 @@ csg_status csg_system_configure(csg_ctx* c, const csg_potential* pot, double ran
     L.base = S->scratch.p;
     S->layout = L;
     S->scratchBlocks = blocks;
+    csg_status st = sizeLastTier(c, S, std::max(c->maximumDistance, range));
+    if (st) return st;
     S->configured = true;
     return CSG_OK;
 }
@@ -890,6 +1006,10 @@ csg_status csg_system_compute_forces(csg_ctx* c, double* forces, double* energy)
     if (!S || !S->configured) return fail(c, CSG_ERR_STATE, "csg_system_init and csg_system_configure first");
     HIPCHECK(c, hipSetDevice(c->device));
     csg_status s = sysForces(c, S);
+    if (s) return s;
+    s = resolveLeftovers(c, S);
+    if (s) return s;
+    s = checkCounters(c, S);
     if (s) return s;
     if (forces) HIPCHECK(c, hipMemcpyAsync(forces, S->force.p, 24 * (size_t)S->N, hipMemcpyDeviceToHost, c->stream));
     if (energy) {
@@ -926,8 +1046,7 @@ csg_status csg_system_move(csg_ctx* c, const double* disp, int32_t transportVelo
     HIPCHECK(c, launchTransport(T, c->stream));
     hipLaunchKernelGGL(kCountNonzero, dim3(blocksFor(S->N)), dim3(kBlock), 0, c->stream, S->N, S->tstatus.as<int>(), S->counters.as<int>(),
                        (int)CNT_WALK_FAIL, true);
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
-    return CSG_OK;
+    return checkCounters(c, S);
 }
 
 csg_status csg_system_nve(csg_ctx* c, double dt, int32_t nSteps) {
@@ -967,7 +1086,8 @@ csg_status csg_system_nvt(csg_ctx* c, double dt, double T, double tau, int32_t M
 csg_status csg_system_sync(csg_ctx* c) {
     if (!c) return CSG_ERR_ARGUMENT;
     HIPCHECK(c, hipStreamSynchronize(c->stream));
-    return CSG_OK;
+    SystemState* S = sysOf(c);
+    return S ? checkCounters(c, S) : CSG_OK;
 }
 
 csg_status csg_system_get(csg_ctx* c, int32_t* face, double* bary, double* vel, double* forces) {
@@ -1252,7 +1372,6 @@ csg_status csg_system_set_groups(csg_ctx* c, int32_t nGroups, const int32_t* gro
     }
     if (!groupOf || !pots || !ranges || !maximumDistances) return fail(c, CSG_ERR_ARGUMENT, "null group arrays");
     if (!S->cellList) return fail(c, CSG_ERR_STATE, "particle groups need cell-list candidates");
-    if (S->screenOn) return fail(c, CSG_ERR_STATE, "the learned screen is on");
     for (int i = 0; i < S->N; ++i)
         if (groupOf[i] < 0 || groupOf[i] >= nGroups) return fail(c, CSG_ERR_ARGUMENT, "group index out of range");
     for (int g = 0; g < nGroups; ++g)
@@ -1271,11 +1390,19 @@ csg_status csg_system_set_groups(csg_ctx* c, int32_t nGroups, const int32_t* gro
     HIPCHECK(c, S->gmaxd.ensure(8 * (size_t)nGroups));
     HIPCHECK(c, S->grmax.ensure(8 * (size_t)nGroups));
     HIPCHECK(c, S->gsent.ensure(8 * (size_t)nGroups));
+    HIPCHECK(c, S->grange.ensure(8 * (size_t)nGroups));
     HIPCHECK(c, hipMemcpy(S->grp.p, groupOf, 4 * (size_t)S->N, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(S->grange.p, ranges, 8 * (size_t)nGroups, hipMemcpyHostToDevice));
     HIPCHECK(c, hipMemcpy(S->gpot.p, S->hgpot.data(), sizeof(Potential) * (size_t)nGroups, hipMemcpyHostToDevice));
     HIPCHECK(c, hipMemcpy(S->gr2.p, r2.data(), 8 * (size_t)nGroups, hipMemcpyHostToDevice));
     HIPCHECK(c, hipMemcpy(S->gmaxd.p, maximumDistances, 8 * (size_t)nGroups, hipMemcpyHostToDevice));
     S->nGroups = nGroups;
+    double radius = std::max(c->maximumDistance, S->range);
+    for (int g = 0; g < nGroups; ++g) radius = std::max(radius, std::max(ranges[g], maximumDistances[g]));
+    if (radius > S->bigRadius) {
+        csg_status st = sizeLastTier(c, S, radius);
+        if (st) return st;
+    }
     return uploadGroupRadii(c, S);
 }
 
@@ -1284,7 +1411,6 @@ csg_status csg_system_set_screen(csg_ctx* c, const float* params, float threshol
     if (!c) return CSG_ERR_ARGUMENT;
     if (!S || !S->configured) return fail(c, CSG_ERR_STATE, "csg_system_init and csg_system_configure first");
     if (params && (!S->cellList || !c->submeshing)) return fail(c, CSG_ERR_STATE, "the screen needs cell-list candidates and submeshing");
-    if (params && S->nGroups) return fail(c, CSG_ERR_STATE, "the screen is unavailable while particle groups are set");
     dropGraph(S);
     HIPCHECK(c, hipSetDevice(c->device));
     if (!params) {
@@ -1292,8 +1418,7 @@ csg_status csg_system_set_screen(csg_ctx* c, const float* params, float threshol
         return CSG_OK;
     }
     const size_t cap = (size_t)std::max<long>(S->candCap, 1);
-    HIPCHECK(c, S->sfeat.ensure(sizeof(float) * kScreenIn * cap));
-    HIPCHECK(c, S->slogit.ensure(sizeof(float) * cap));
+    HIPCHECK(c, S->skeep.ensure(cap));
     HIPCHECK(c, S->scand.ensure(4 * cap));
     HIPCHECK(c, S->scnt.ensure(4 * ((size_t)S->N + 1)));
     HIPCHECK(c, S->soff.ensure(4 * ((size_t)S->N + 1)));

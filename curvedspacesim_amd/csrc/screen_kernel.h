@@ -18,6 +18,7 @@ constexpr int kScreenW3 = kScreenB2 + kScreenHidden;
 constexpr int kScreenB3 = kScreenW3 + kScreenHidden;
 constexpr int kScreenParams = kScreenB3 + 1;  // 5,313
 constexpr int kScreenThreads = 256;           // 4 wavefronts, 16 samples per wavefront per tile
+constexpr int kScreenPersistentBlocks = 1024; // inference workgroups: four per CU, each wavefront looping over tiles or sources
 // Pairs whose first feature, 100 (1 - |d| / rc), exceeds this are always kept, whatever their score: such a pair lies
 // beyond the cutoff only if its geodesic exceeds the chord by more than 1/0.985 - 1 = 1.5%, over twice the largest
 // excess of any candidate pair of the benchmark surfaces (0.71%).
@@ -46,9 +47,15 @@ struct ScreenFeatureParams {
     const int32_t* tidx;
     const int32_t* patchCount;  // faces of each source's patch (nullable)
     double cutoff;
+    const int32_t* srcGroup;    // particle groups (nullable): the cutoff of source i is groupRange[srcGroup[i]]
+    const double* groupRange;
     const int* nPairsDev;  // optional: pair count on the device (the launch size bounds it)
     float* out;
 };
 hipError_t launchScreenFeatures(const ScreenFeatureParams& P, int nPairs, hipStream_t stream);
+// the screening of a force computation: per candidate pair a keep flag (pairs closer than the guard distance are kept
+// unscored; the others are scored and kept when their logit reaches the threshold) and per source the kept count
+hipError_t launchScreenSources(const ScreenFeatureParams& P, const float* params, float threshold, uint8_t* keep, int32_t* count,
+                               hipStream_t stream);
 
 }  // namespace csg

@@ -1,11 +1,22 @@
-// Learned pair screen on gfx950: MLP 16 -> 64 -> 64 -> 1 with ReLU, BF16 operands on the matrix cores
-// (v_mfma_f32_16x16x16_bf16), FP32 accumulation, FP32 master weights. No reference counterpart.
+// Learned pair screen on gfx950: MLP 16 -> 64 -> 64 -> 1 with ReLU, BF16 operands on the matrix cores, FP32
+// accumulation, FP32 master weights. No reference counterpart.
 //
-// A workgroup of four wavefronts takes 64 samples per iteration, 16 per wavefront. For a 16 x 16 x 16 product a lane l
-// (r = l % 16, g = l / 16) holds A[r][4g .. 4g+3] and B[4g .. 4g+3][r] and receives D[4g .. 4g+3][r]; a layer output
-// in that D layout is moved through LDS to become the A operand of the next layer. The backward pass reuses the D
-// layouts directly: H1 in D layout is the A operand of H1^T dH2, dH2 and dH1 in D layout are B operands.
+// Inference (screenForwardKernel, screenSourcesKernel) runs the layers transposed, H^T = W^T X^T, on
+// v_mfma_f32_16x16x32_bf16: the weights are the A operand (a 16 x 32 block: lane l, r = l % 16, g = l / 16, holds
+// A[r][8g .. 8g+7]), the 16 samples of a tile are the columns of the B operand (lane l holds B[8g .. 8g+7][r]) and of
+// the result (lane l receives D[4g .. 4g+3][r]). Layer 1's output tile j holds hidden units 16j .. 16j+15; layer 2
+// reads its K dimension in the order in which those outputs already lie in the lanes (K position 8g+t of chunk kb is
+// hidden unit 16 (2kb + t/4) + 4g + t%4), so the B operand of layer 2 is the lane's own layer-1 results: no exchange
+// between lanes and no barrier. Each wavefront keeps its weight fragments in registers for all its tiles.
+//
+// Training (screenGradientKernel) uses v_mfma_f32_16x16x16_bf16 in the untransposed orientation: a workgroup of four
+// wavefronts takes 64 samples per iteration, 16 per wavefront; a lane holds A[r][4g .. 4g+3] and B[4g .. 4g+3][r] and
+// receives D[4g .. 4g+3][r]; a layer output in that D layout is moved through LDS to become the A operand of the next
+// layer, and the backward pass reuses the D layouts directly (H1 in D layout is the A operand of H1^T dH2, dH2 and dH1
+// in D layout are B operands).
 #include <math.h>
+
+#include <algorithm>
 
 #include "screen_kernel.h"
 
@@ -14,6 +25,7 @@ namespace csg {
 namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int H = kScreenHidden;
@@ -26,6 +38,9 @@ __device__ inline short toBf16(float x) {
 }
 
 __device__ inline f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+__device__ inline f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+__device__ inline __bf16 bf(float x) { return __builtin_bit_cast(__bf16, toBf16(x)); }
+__device__ inline __bf16 bfBits(short v) { return __builtin_bit_cast(__bf16, v); }
 
 // weights staged in LDS once per workgroup: W1 and W2 as bfloat16, biases and W3 as float
 struct Staged {
@@ -96,26 +111,84 @@ __device__ void loadTile(const float* X, int n, int t0, float* xs) {
     }
 }
 
+// A wavefront's register copy of the network for the transposed layers (see the top of this file): lane (r, g) holds
+//   w1[j]      A of layer 1, tile j:           element t = W1[8g + t][16j + r]  (zero for g >= 2: features 16 .. 31)
+//   w2[j][kb]  A of layer 2, tile j, chunk kb: element t = W2[16 (2kb + t/4) + 4g + t%4][16j + r]
+//   b1, b2, w3 [tile][i]: the values of hidden unit 16 tile + 4g + i, the rows of the lane's results
+struct Frags {
+    bf16x8 w1[4];
+    bf16x8 w2[4][2];
+    f32x4 b1[4], b2[4], w3[4];
+    float b3;
+};
+
+__device__ void loadFrags(const Staged& s, Frags& F) {
+    const int l = threadIdx.x & 63, r = l & 15, g = l >> 4;
+    for (int j = 0; j < 4; ++j) {
+        for (int t = 0; t < 8; ++t) F.w1[j][t] = bfBits(g < 2 ? s.w1[(8 * g + t) * H + 16 * j + r] : (short)0);
+        for (int kb = 0; kb < 2; ++kb)
+            for (int t = 0; t < 8; ++t) F.w2[j][kb][t] = bfBits(s.w2[(16 * (2 * kb + t / 4) + 4 * g + t % 4) * H + 16 * j + r]);
+        for (int i = 0; i < 4; ++i) {
+            F.b1[j][i] = s.b1[16 * j + 4 * g + i];
+            F.b2[j][i] = s.b2[16 * j + 4 * g + i];
+            F.w3[j][i] = s.w3[16 * j + 4 * g + i];
+        }
+    }
+    F.b3 = s.b3;
+}
+
+// Logit of sample r of a 16-sample tile whose features lane (r, g) holds in x (features 8g .. 8g+7, zero for g >= 2).
+// Every lane of column r returns it. Operands are rounded to bfloat16 as in the layer-by-layer form: features,
+// weights and the layer-1 activations; products accumulate in FP32.
+__device__ inline float mlpTile(const Frags& F, bf16x8 x) {
+    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+    f32x4 h1[4];
+    for (int j = 0; j < 4; ++j) {
+        h1[j] = mfma32(F.w1[j], x, zero);
+        for (int i = 0; i < 4; ++i) h1[j][i] = fmaxf(h1[j][i] + F.b1[j][i], 0.f);
+    }
+    f32x4 h2[4] = {zero, zero, zero, zero};
+    for (int kb = 0; kb < 2; ++kb) {
+        bf16x8 b;
+        for (int t = 0; t < 4; ++t) {
+            b[t] = bf(h1[2 * kb][t]);
+            b[4 + t] = bf(h1[2 * kb + 1][t]);
+        }
+        for (int j = 0; j < 4; ++j) h2[j] = mfma32(F.w2[j][kb], b, h2[j]);
+    }
+    float z = 0.f;
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 4; ++i) z += fmaxf(h2[j][i] + F.b2[j][i], 0.f) * F.w3[j][i];
+    z += __shfl_xor(z, 16, 64);
+    z += __shfl_xor(z, 32, 64);
+    return z + F.b3;
+}
+
+// Logits of n feature rows. Persistent wavefronts: one barrier to stage the weights, then each wavefront loads its
+// fragments once and takes 16-row tiles independently (tile t by wavefront t mod the wavefront count).
 __global__ __launch_bounds__(kScreenThreads) void screenForwardKernel(const float* params, const float* X, int n, const int* nDev,
                                                                      float* logits) {
     __shared__ Staged s;
-    __shared__ float xs[64 * kScreenIn];
-    __shared__ float tr[4][16 * H];
-    if (nDev) n = *nDev;
+    if (nDev) n = min(n, *nDev);
     stageWeights(params, s);
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, g = l >> 4;
-    for (int t0 = blockIdx.x * 64; t0 < n; t0 += gridDim.x * 64) {
-        __syncthreads();
-        loadTile(X, n, t0, xs);
-        __syncthreads();
-        f32x4 h1[4], h2[4];
-        float z[4];
-        forwardTile(s, xs + w * 16 * kScreenIn, tr[w], h1, h2, z);
-        if (r == 0)
-            for (int i = 0; i < 4; ++i) {
-                const int smp = t0 + 16 * w + 4 * g + i;
-                if (smp < n) logits[smp] = z[i];
-            }
+    __syncthreads();
+    Frags F;
+    loadFrags(s, F);
+    const int l = threadIdx.x & 63, r = l & 15, g = l >> 4;
+    const int waves = gridDim.x * (kScreenThreads / 64);
+    const int tiles = (n + 15) / 16;
+    for (int t = blockIdx.x * (kScreenThreads / 64) + (threadIdx.x >> 6); t < tiles; t += waves) {
+        const int row = 16 * t + r;
+        bf16x8 x;
+        if (g < 2 && row < n) {
+            const float4* p = reinterpret_cast<const float4*>(X + (size_t)row * kScreenIn + 8 * g);
+            const float4 a = p[0], b = p[1];
+            x = bf16x8{bf(a.x), bf(a.y), bf(a.z), bf(a.w), bf(b.x), bf(b.y), bf(b.z), bf(b.w)};
+        } else {
+            for (int k = 0; k < 8; ++k) x[k] = bfBits(0);
+        }
+        const float z = mlpTile(F, x);
+        if (g == 0 && row < n) logits[row] = z;
     }
 }
 
@@ -248,12 +321,22 @@ __global__ __launch_bounds__(kScreenThreads) void screenGradientKernel(const flo
     for (int i = threadIdx.x; i <= kScreenParams; i += blockDim.x) out[i] = acc[i];
 }
 
-__global__ void screenReduceKernel(const float* partial, int blocks, float* grad) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > kScreenParams) return;
-    float s = 0.f;
-    for (int b = 0; b < blocks; ++b) s += partial[(size_t)b * (kScreenParams + 1) + p];
-    grad[p] = s;
+// fixed-order sum of the workgroups' partial gradients: a workgroup per 64 parameters, four lanes per parameter summing
+// every fourth partial row with several loads in flight, then the four in a fixed order
+__global__ __launch_bounds__(256) void screenReduceKernel(const float* partial, int blocks, float* grad) {
+    __shared__ float part[4][64];
+    const int pl = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + pl;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    if (p <= kScreenParams) {
+        int b = q;
+        for (; b + 12 < blocks; b += 16)
+            for (int u = 0; u < 4; ++u) acc[u] += partial[(size_t)(b + 4 * u) * (kScreenParams + 1) + p];
+        for (; b < blocks; b += 4) acc[0] += partial[(size_t)b * (kScreenParams + 1) + p];
+    }
+    part[q][pl] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    __syncthreads();
+    if (q == 0 && p <= kScreenParams) grad[p] = (part[0][pl] + part[1][pl]) + (part[2][pl] + part[3][pl]);
 }
 
 __global__ void screenAdamKernel(float* params, const float* grad, float* m, float* v, int step, float lr) {
@@ -270,17 +353,11 @@ __global__ void screenAdamKernel(float* params, const float* grad, float* m, flo
     params[p] -= lr * mh / (sqrtf(vh) + eps);
 }
 
-// features of pair q: its source by binary search over the offsets
-__global__ void screenFeaturesKernel(ScreenFeatureParams P, int nPairs) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nPairs || (P.nPairsDev && q >= *P.nPairsDev)) return;
-    int lo = 0, hi = P.nSrc - 1;
-    while (lo < hi) {  // last source whose first pair is at or before q
-        const int mid = (lo + hi + 1) >> 1;
-        if (P.toff[mid] <= q) lo = mid;
-        else hi = mid - 1;
-    }
-    const int i = lo, j = P.tidx[q];
+// the cutoff of source i: its group's range, or the system's
+__device__ inline double sourceCutoff(const ScreenFeatureParams& P, int i) { return P.srcGroup ? P.groupRange[P.srcGroup[i]] : P.cutoff; }
+
+// the 16 features of the pair (source i, target j) with cutoff rc
+__device__ void pairFeatures(const ScreenFeatureParams& P, int i, int j, double rc, float* o) {
     const DeviceMesh& M = P.mesh;
     const int fi = P.face[i], fj = P.face[j];
     const double* ni = M.fnrm + 3 * fi;
@@ -299,11 +376,9 @@ __global__ void screenFeaturesKernel(ScreenFeatureParams P, int nPairs) {
     }
     double d[3] = {P.xyz[3 * j] - P.xyz[3 * i], P.xyz[3 * j + 1] - P.xyz[3 * i + 1], P.xyz[3 * j + 2] - P.xyz[3 * i + 2]};
     const double dE = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    const double rc = P.cutoff;
     const double riseI = d[0] * ni[0] + d[1] * ni[1] + d[2] * ni[2];  // rise of the chord above the source's plane
     const double riseJ = d[0] * nj[0] + d[1] * nj[1] + d[2] * nj[2];
     const double dd = dE > 0 ? dE : 1e-300;
-    float* o = P.out + (size_t)q * kScreenIn;
     // the distance as its margin to the cutoff in percent, so that bfloat16 resolves it near the cutoff
     o[0] = (float)(100.0 * (1.0 - dE / rc));
     o[1] = (float)(ni[0] * nj[0] + ni[1] * nj[1] + ni[2] * nj[2]);
@@ -323,13 +398,100 @@ __global__ void screenFeaturesKernel(ScreenFeatureParams P, int nPairs) {
     o[15] = (float)(10.0 * riseI / dd);
 }
 
+// the first feature alone: 100 (1 - |x_j - x_i| / rc), as pairFeatures computes it
+__device__ inline float cutoffMargin(const ScreenFeatureParams& P, int i, int j, double rc) {
+    const double d0 = P.xyz[3 * j] - P.xyz[3 * i], d1 = P.xyz[3 * j + 1] - P.xyz[3 * i + 1], d2 = P.xyz[3 * j + 2] - P.xyz[3 * i + 2];
+    return (float)(100.0 * (1.0 - sqrt(d0 * d0 + d1 * d1 + d2 * d2) / rc));
+}
+
+// features of pair q: its source by binary search over the offsets
+__global__ void screenFeaturesKernel(ScreenFeatureParams P, int nPairs) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nPairs || (P.nPairsDev && q >= *P.nPairsDev)) return;
+    int lo = 0, hi = P.nSrc - 1;
+    while (lo < hi) {  // last source whose first pair is at or before q
+        const int mid = (lo + hi + 1) >> 1;
+        if (P.toff[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    float o[kScreenIn];
+    pairFeatures(P, lo, P.tidx[q], sourceCutoff(P, lo), o);
+    float4* out = reinterpret_cast<float4*>(P.out + (size_t)q * kScreenIn);
+    for (int k = 0; k < 4; ++k) out[k] = float4{o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]};
+}
+
+// Screening of the step, one source per wavefront pass (persistent wavefronts): every candidate closer than the guard
+// distance is kept without scoring; the others (the guard band, a few per source) get their features and their logit,
+// 16 to a tile, and are kept when it reaches the threshold. Writes the keep flag of every candidate pair and the kept
+// count of every source. A wavefront's band pairs pass their features to the lanes of the B operand through 1 KB of
+// wavefront-private LDS; the only workgroup barrier is the one after the weights are staged.
+__global__ __launch_bounds__(kScreenThreads) void screenSourcesKernel(ScreenFeatureParams P, const float* params, float threshold,
+                                                                     uint8_t* keep, int32_t* count) {
+    __shared__ Staged s;
+    __shared__ float rows[kScreenThreads / 64][16 * kScreenIn];
+    stageWeights(params, s);
+    __syncthreads();
+    Frags F;
+    loadFrags(s, F);
+    const int l = threadIdx.x & 63, r = l & 15, g = l >> 4, w = threadIdx.x >> 6;
+    float* xs = rows[w];
+    const unsigned long long below = (1ull << l) - 1ull;
+    const int waves = gridDim.x * (kScreenThreads / 64);
+    for (int i = blockIdx.x * (kScreenThreads / 64) + w; i < P.nSrc; i += waves) {
+        const double rc = sourceCutoff(P, i);
+        const int b0 = P.toff[i], b1 = P.toff[i + 1];
+        int kept = 0;
+        for (int c0 = b0; c0 < b1; c0 += 64) {
+            const int q = c0 + l;
+            const bool valid = q < b1;
+            const int j = valid ? P.tidx[q] : 0;
+            const bool band = valid && cutoffMargin(P, i, j, rc) <= kScreenGuard;
+            bool keepIt = valid;
+            unsigned long long pending = __ballot(band);
+            while (pending) {
+                const int rank = __popcll(pending & below);
+                const bool mine = ((pending >> l) & 1ull) && rank < 16;
+                const int nRows = min(16, __popcll(pending));
+                if (mine) {
+                    float o[kScreenIn];
+                    pairFeatures(P, i, j, rc, o);
+                    for (int k = 0; k < kScreenIn; ++k) xs[rank * kScreenIn + k] = o[k];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                bf16x8 x;
+                for (int k = 0; k < 8; ++k) x[k] = (g < 2 && r < nRows) ? bf(xs[r * kScreenIn + 8 * g + k]) : bfBits(0);
+                const float z = mlpTile(F, x);
+                const float zr = __shfl(z, mine ? rank : 0, 64);  // lane `rank` holds the logit of row `rank`
+                if (mine) keepIt = zr >= threshold;
+                pending &= ~__ballot(mine);
+                __builtin_amdgcn_wave_barrier();  // this batch's rows are read before the next batch writes them
+            }
+            if (valid) keep[q] = keepIt ? 1 : 0;
+            kept += __popcll(__ballot(keepIt));
+        }
+        if (l == 0) count[i] = kept;
+    }
+}
+
 }  // namespace
 
 int screenTrainBlocks(int n) { return n <= 0 ? 1 : (n + 63) / 64 < 1024 ? (n + 63) / 64 : 1024; }
 
 hipError_t launchScreenForward(const float* params, const float* X, int n, const int* nDev, float* logits, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(screenForwardKernel, dim3(screenTrainBlocks(n)), dim3(kScreenThreads), 0, stream, params, X, n, nDev, logits);
+    const long tiles = ((long)n + 15) / 16;
+    const int blocks = (int)std::min<long>((tiles + 3) / 4, kScreenPersistentBlocks);
+    hipLaunchKernelGGL(screenForwardKernel, dim3(blocks), dim3(kScreenThreads), 0, stream, params, X, n, nDev, logits);
+    return hipGetLastError();
+}
+
+hipError_t launchScreenSources(const ScreenFeatureParams& P, const float* params, float threshold, uint8_t* keep, int32_t* count,
+                               hipStream_t stream) {
+    if (P.nSrc <= 0) return hipSuccess;
+    const int blocks = std::min<int>((P.nSrc + 3) / 4, kScreenPersistentBlocks);
+    hipLaunchKernelGGL(screenSourcesKernel, dim3(blocks), dim3(kScreenThreads), 0, stream, P, params, threshold, keep, count);
     return hipGetLastError();
 }
 
@@ -338,7 +500,7 @@ hipError_t launchScreenGradient(const float* params, const float* X, const float
     if (n <= 0) return hipErrorInvalidValue;
     const int blocks = screenTrainBlocks(n);
     hipLaunchKernelGGL(screenGradientKernel, dim3(blocks), dim3(kScreenThreads), 0, stream, params, X, y, n, partial);
-    hipLaunchKernelGGL(screenReduceKernel, dim3((kScreenParams + 256) / 256), dim3(256), 0, stream, partial, blocks, grad);
+    hipLaunchKernelGGL(screenReduceKernel, dim3((kScreenParams + 64) / 64), dim3(256), 0, stream, partial, blocks, grad);
     return hipGetLastError();
 }
 

@@ -25,15 +25,34 @@ def init_params(seed):
     return p
 
 
-def collect(ctx, w):
-    """Features and labels (1: the exact geodesic distance is below the range) of every candidate pair of one force
-    computation of workload w, plus the system."""
+def collect(ctx, w, md_steps=(0,)):
+    """Features and labels (1: the exact geodesic distance is below the range) of every candidate pair of workload w,
+    taken after each of the given numbers of its own MD steps from the placement (pairs of all snapshots stacked),
+    plus the system. The screen meets states that the dynamics evolved, whose pair statistics differ from those of
+    a fresh placement (repulsion thins out close pairs), so its data come from such states too."""
     sysm = workloads.make_system(ctx, w)
-    sysm.record_pairs(True)
-    sysm.compute_forces()
-    d = sysm.pair_distances()
-    X = sysm.screen_features()
-    return X, (d < w.range).astype(np.float32), sysm
+    Xs, ys = [], []
+    done = 0
+    for n in sorted(md_steps):
+        if n > done:
+            workloads.step(sysm, w, n - done)
+            done = n
+        sysm.record_pairs(True)
+        sysm.compute_forces()
+        d = sysm.pair_distances()
+        Xs.append(sysm.screen_features())
+        sysm.record_pairs(False)
+        ys.append((d < pair_ranges(sysm, w)).astype(np.float32))
+    return np.concatenate(Xs), np.concatenate(ys), sysm
+
+
+def pair_ranges(sysm, w):
+    """The cutoff of every candidate pair of the last force computation: the range, or the source's group range."""
+    groups = w.meta.get("groups")
+    if groups is None:
+        return np.float64(w.range)
+    off, _ = sysm.pairs()
+    return np.repeat(np.asarray(w.meta["ranges"], float)[groups], np.diff(off))
 
 
 def balance(X, y, fraction=0.25):
@@ -60,27 +79,32 @@ def dropped(ctx, params, threshold, X):
     return (ctx.screen_forward(params, X) < threshold) & (X[:, 0] <= GUARD)
 
 
-def calibrate(ctx, params, X, y, margin=8.0):
+def calibrate(ctx, params, X, y, margin=12.0):
     """Threshold below every within-range pair's logit of the held-out rows, by `margin` (logit units)."""
     z = ctx.screen_forward(params, X)
     pos = z[y > 0.5]
     return float(pos.min() - margin) if len(pos) else float(z.min() - margin)
 
 
+TRAIN_MD_STEPS = (0, 40, 80)
+HOLDOUT_MD_STEPS = (25, 50, 100, 150)
+
+
 def build(ctx, config=3, n=None, train_seeds=(13378, 13379), holdout_seeds=(13380, 13381), steps=2000, batch=8192, lr=2e-3,
-          margin=8.0):
+          margin=12.0, train_md_steps=TRAIN_MD_STEPS, holdout_md_steps=HOLDOUT_MD_STEPS):
     """Train and calibrate a screen on systems of the given configuration with other seeds than the one it will
-    screen. Returns dict(params, threshold, losses, train_pairs, holdout_pairs, holdout_drop, holdout_beyond)."""
+    screen, from snapshots taken after the given numbers of MD steps. Returns dict(params, threshold, losses,
+    train_pairs, holdout_pairs, holdout_drop, holdout_beyond)."""
     Xs, ys = [], []
     for s in train_seeds:
-        X, y, _ = collect(ctx, workloads.config(config, seed=s, n=n))
+        X, y, _ = collect(ctx, workloads.config(config, seed=s, n=n), train_md_steps)
         Xs.append(X)
         ys.append(y)
     X = np.concatenate(Xs)
     y = np.concatenate(ys)
     Xt, yt = balance(X, y)
     params, losses = train(ctx, Xt, yt, steps=steps, batch=batch, lr=lr)
-    held = [collect(ctx, workloads.config(config, seed=s, n=n))[:2] for s in holdout_seeds]
+    held = [collect(ctx, workloads.config(config, seed=s, n=n), holdout_md_steps)[:2] for s in holdout_seeds]
     Xh = np.concatenate([h[0] for h in held])
     yh = np.concatenate([h[1] for h in held])
     thr = calibrate(ctx, params, Xh, yh, margin)

@@ -115,7 +115,11 @@ csg_status csg_positions(csg_ctx* ctx, int32_t n, const int32_t* face, const dou
 /* --- device-resident particle system: the batched seam of simpleModel (src/models/simpleModel.h:25-111) ----------
  * Positions, velocities, masses and forces live in HBM; every step is a fixed sequence of launches on the context
  * stream with no host synchronisation. Errors inside a step (candidate capacity, an unsolved source, a failed walk)
- * are counted on the device and reported by csg_system_status. */
+ * are counted on the device (csg_system_status) and reported by the next synchronous call that checks them
+ * (csg_system_compute_forces, csg_system_move, csg_system_sync): CSG_ERR_WALK when walks failed since the last check
+ * (those particles kept their positions, as the reference throws at triangulatedMeshSpace.cpp:522-548), and
+ * CSG_ERR_CAPACITY when a source lost candidates to the candidate capacity or got zero force because it exceeded the
+ * step's geodesic tables. */
 /* simpleModel::initializeSimpleModel / setParticlePositions (simpleModel.cpp:7-35, :318-326); vel and mass nullable
  * (zero velocities, unit masses) */
 csg_status csg_system_init(csg_ctx* ctx, int32_t N, const int32_t* face, const double* bary, const double* vel, const double* mass);
@@ -125,16 +129,20 @@ csg_status csg_system_init(csg_ctx* ctx, int32_t N, const int32_t* face, const d
  * HBM tables for sources the LDS tables cannot hold (0: 1536 faces; at most 4096 targets per source with a cell
  * list). Uses the context's submeshing setting. */
 csg_status csg_system_configure(csg_ctx* ctx, const csg_potential* pot, double range, int32_t cellList, double rMax, int32_t maxPatchFaces);
-/* simulation::computeForces (simulation.cpp:61-70): findNeighbors + pair forces; forces (3N) and energy nullable */
+/* simulation::computeForces (simulation.cpp:61-70): findNeighbors + pair forces; forces (3N) and energy nullable.
+ * Synchronous: a source that the step's fixed tables cannot hold is solved again with HBM tables that grow until it
+ * fits (as csg_pair_forces does); CSG_ERR_CAPACITY only if one still does not fit or candidates overflowed. */
 csg_status csg_system_compute_forces(csg_ctx* ctx, double* forces, double* energy);
-/* simpleModel::moveParticles (simpleModel.cpp:44-66) with host displacements (3N); velocities transported if asked */
+/* simpleModel::moveParticles (simpleModel.cpp:44-66) with host displacements (3N); velocities transported if asked;
+ * CSG_ERR_WALK if a walk failed */
 csg_status csg_system_move(csg_ctx* ctx, const double* disp, int32_t transportVelocity);
-/* nSteps of velocityVerletNVE::performUpdate (velocityVerletNVE.cpp:3-29), asynchronous */
+/* nSteps of velocityVerletNVE::performUpdate (velocityVerletNVE.cpp:3-29), asynchronous: errors are reported by the
+ * next csg_system_sync */
 csg_status csg_system_nve(csg_ctx* ctx, double dt, int32_t nSteps);
 /* nSteps of noseHooverNVT::performUpdate (noseHooverNVT.cpp:42-139) with chain length M, asynchronous; the chain is
  * initialised on the first call (setBathVariables, :28-36) and kept across calls with the same (T, tau, M) */
 csg_status csg_system_nvt(csg_ctx* ctx, double dt, double T, double tau, int32_t M, int32_t nSteps);
-/* waits for the context stream */
+/* waits for the context stream, then reports the errors counted since the last check (see above) */
 csg_status csg_system_sync(csg_ctx* ctx);
 /* copies state back (every pointer nullable): faces N, barycentric 3N, velocities 3N, forces 3N */
 csg_status csg_system_get(csg_ctx* ctx, int32_t* face, double* bary, double* vel, double* forces);
@@ -189,7 +197,8 @@ csg_status csg_system_pairs(csg_ctx* ctx, int32_t* offsets, int32_t* idx, int64_
  * size) and maximum distance (patch-radius cap and sentinel, as csg_set_submeshing's for a single system). Particles of
  * different groups are never candidates of each other, so groups on disjoint meshes of one uploaded union do not
  * interact. A group's range is also its truncation radius (raised as rMax is for a single system). Requires the cell
- * list; the learned screen is unavailable while groups are set. nGroups = 0 clears, and so does csg_system_configure. */
+ * list. With the learned screen on, a pair's features and guard distance use its group's range. nGroups = 0 clears,
+ * and so does csg_system_configure. */
 csg_status csg_system_set_groups(csg_ctx* ctx, int32_t nGroups, const int32_t* groupOf, const csg_potential* pots, const double* ranges,
                                  const double* maximumDistances);
 /* keep the geodesic distance of every candidate pair of later force computations (no reference counterpart; the
@@ -220,8 +229,9 @@ csg_status csg_screen_train(csg_ctx* ctx, float* params, int32_t n, const float*
 /* features of the candidate pairs of the last force computation (cell list and submeshing required) */
 csg_status csg_system_screen_features(csg_ctx* ctx, float* features, int64_t cap);
 /* screen later force computations: pairs whose logit is below threshold and whose Euclidean separation is at least
- * 0.985 rc are not solved (params NULL: screen off); closer pairs are always kept. Patches are still built from every
- * candidate, so every kept pair gets the distance of the plain computation. */
+ * 0.985 rc (rc: the range, or the pair's group range) are not solved (params NULL: screen off); closer pairs are kept
+ * without being scored. Patches are still built from every candidate, so every kept pair gets the distance of the plain
+ * computation. */
 csg_status csg_system_set_screen(csg_ctx* ctx, const float* params, float threshold);
 /* candidate pairs and kept pairs of the last force computation */
 csg_status csg_system_screen_stats(csg_ctx* ctx, int64_t* candidates, int64_t* kept);

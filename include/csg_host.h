@@ -1,6 +1,7 @@
-/* C facade of the C++ host layer (libcsg_host.so): workload meshes and whole-simulation drivers built from the
- * reference-API classes (simpleModel, force, updater, simulation). The reference has no C interface for these; the
- * facade exists so that scripting callers (the Python tests and the benchmark) can drive the C++ classes.
+/* C facade of the C++ host layer (libcsg_host.so): the meshes of the workloads (generators and an OFF reader). The
+ * reference-API classes of that library (simpleModel, force, updater, simulation and the gpu* adapters) are C++ and
+ * are used directly from C++ (tests/cpp/refapi_scenarios.cpp); the reference has no C interface for meshes, and this
+ * facade exists so that scripting callers (the Python tests and the benchmark) can build them.
  * All functions return 0 on success and a negative value on failure; csgh_last_error() then describes it.
  */
 #ifndef CSG_HOST_H

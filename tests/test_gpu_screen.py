@@ -73,6 +73,42 @@ def test_training_reduces_loss(ctx, rng):
     assert acc > 0.95
 
 
+@pytest.fixture(scope="module")
+def full_screen(ctx):
+    """The screen of the benchmark: trained on MD snapshots of two other N = 20,000 bumpy spheres (seeds 13378, 13379),
+    calibrated on snapshots of two more (13380, 13381)."""
+    print("case build the N = 20,000 screen", flush=True)
+    return screen.build(ctx, config=3)
+
+
+def test_screened_forces_match_plain_full_size_md(ctx, full_screen):
+    """The benchmark workload (N = 20,000, seed 13377) after 25, 50 and 100 Nose-Hoover steps from its placement: no
+    pair within the cutoff is dropped and the screened forces equal the plain ones."""
+    w = workloads.config(3)
+    sysm = workloads.make_system(ctx, w)
+    done = 0
+    for n in (25, 50, 100):
+        workloads.step(sysm, w, n - done)
+        done = n
+        print(f"case screened vs plain, N = 20,000 after {n} NVT steps", flush=True)
+        sysm.record_pairs(True)
+        F0, E0 = sysm.compute_forces()
+        d = sysm.pair_distances()
+        X = sysm.screen_features()
+        sysm.record_pairs(False)
+        drop = screen.dropped(ctx, full_screen["params"], full_screen["threshold"], X)
+        assert not np.any(drop & (d < w.range)), f"{int(np.sum(drop & (d < w.range)))} within-cutoff pairs would be dropped"
+        sysm.set_screen(full_screen["params"], full_screen["threshold"])
+        F1, E1 = sysm.compute_forces()
+        st = sysm.screen_stats()
+        sysm.set_screen(None)
+        assert st["candidates"] == len(d) and st["kept"] == int(np.sum(~drop))
+        assert np.max(np.abs(F1 - F0)) <= 1e-6 * np.abs(F0).max()
+        assert abs(E1 - E0) <= 1e-12 * abs(E0)
+        print(f"screen after {n} steps: {int(drop.sum())} of {len(d)} pairs dropped, beyond cutoff {np.mean(d >= w.range):.4%}",
+              flush=True)
+
+
 def test_screened_forces_match_plain(ctx):
     """A screen trained on other bumpy spheres (seeds 13378, 13379), calibrated on two more (13380, 13381), screening the
     N = 4,000 bumpy sphere of seed 13377: the forces equal the plain ones, no within-cutoff pair is dropped, and the
