@@ -1,4 +1,5 @@
-"""Builds the native libraries in-tree: libcsg.so (HIP kernels + C-ABI, gfx950) and libcsg_host.so (C++ host layer)."""
+"""Builds the native libraries in-tree: libcsg.so (HIP kernels + C-ABI, gfx950), libcsg_host.so (C++ host layer) and,
+where the HDF5 C library is found, libcsg_h5.so (trajectory files in the reference's simpleModelDatabase layout)."""
 import os
 import subprocess
 import sys
@@ -45,6 +46,32 @@ def build_host():
     return out
 
 
+# the HDF5 C library: CSG_HDF5_DIR, else the image's conda prefix (HDF5 1.10)
+HDF5_DIR = os.environ.get("CSG_HDF5_DIR", "/opt/conda")
+
+
+def hdf5_found():
+    return os.path.exists(os.path.join(HDF5_DIR, "include", "hdf5.h")) and os.path.exists(os.path.join(HDF5_DIR, "lib", "libhdf5.so"))
+
+
+def build_h5():
+    """libcsg_h5.so, or None (after removing a stale one) when the HDF5 C library is not found."""
+    out = os.path.join(PKG, "libcsg_h5.so")
+    if not hdf5_found():
+        if os.path.exists(out):
+            os.remove(out)
+        print(f"HDF5 C library not found under {HDF5_DIR}: libcsg_h5.so (HDF5 trajectory files) not built")
+        return None
+    srcs = [os.path.join(CSRC, "host", f) for f in ("simpleModelDatabase.cpp", "h5Capi.cpp")]
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           "-I" + os.path.join(HDF5_DIR, "include")] + srcs + \
+          ["-L" + PKG, "-lcsg_host", "-lcsg", "-L" + os.path.join(HDF5_DIR, "lib"), "-lhdf5",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(HDF5_DIR, "lib"), "-o", out + ".tmp"]
+    _run(cmd, CSRC)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_scenarios():
     out = os.path.join(ROOT, "tests", "cpp", "refapi_scenarios")
     cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(CSRC, "host"), SCENARIOS, "-L" + PKG, "-lcsg_host", "-lcsg",
@@ -57,9 +84,10 @@ def build_scenarios():
 def build_all(verbose=False):
     a = build_csg()
     b = build_host()
+    h = build_h5()
     c = build_scenarios()
     if verbose:
-        print("built", a, b, c)
+        print("built", a, b, c, h or "(no HDF5 library)")
     return a, b, c
 
 

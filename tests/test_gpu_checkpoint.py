@@ -7,16 +7,21 @@ import numpy as np
 import pytest
 
 import curvedspacesim_amd as csa
-from curvedspacesim_amd import workloads
+from curvedspacesim_amd import trajectory, workloads
 from oracle.pyoracle import OracleMesh
 
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("fmt", ["npz", "h5"])
 @pytest.mark.parametrize("integrator", ["nve", "nvt"])
-def test_resume_continues_the_trajectory(tmp_path, integrator):
+def test_resume_continues_the_trajectory(tmp_path, integrator, fmt):
+    """The checkpoint goes through a file of either format: an .npz archive, or an HDF5 file in the reference's
+    simpleModelDatabase layout (plus the Nose-Hoover chain datasets)."""
+    if fmt == "h5" and not trajectory.available():
+        pytest.skip("libcsg_h5.so not built (no HDF5 C library)")
     w = workloads.config(2, n=2000)
-    print(f"case checkpoint resume {integrator}", flush=True)
+    print(f"case checkpoint resume {integrator} through {fmt}", flush=True)
 
     def run(sysm, steps):
         if integrator == "nve":
@@ -33,9 +38,9 @@ def test_resume_continues_the_trajectory(tmp_path, integrator):
         b = workloads.make_system(cb, w)
         run(b, 3)
         ck = b.checkpoint(time=3 * w.dt)
-        csa.save_checkpoint(tmp_path / "ck.npz", ck)
+        csa.save_checkpoint(tmp_path / f"ck.{fmt}", ck)
         cb.close()
-        back = csa.load_checkpoint(tmp_path / "ck.npz")
+        back = csa.load_checkpoint(tmp_path / f"ck.{fmt}")
         for k in csa._native.CHECKPOINT_FIELDS:
             np.testing.assert_array_equal(back[k], ck[k])
         assert (back["nhParams"][0] == 2) == (integrator == "nvt")
