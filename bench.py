@@ -237,6 +237,9 @@ def main():
     ap.add_argument("--no-screen", action="store_true", help="skip the learned-screen leg")
     ap.add_argument("--no-graph", action="store_true", help="launch every kernel directly instead of replaying a captured step")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the CPU baseline's check of the GPU path")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="first copy 256 MiB between two device buffers (a kernel of known traffic, to scale the HBM "
+                         "byte counters of a profile of this command)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (got --gpus {args.gpus}): this build measures one GPU's share of the "
@@ -246,6 +249,15 @@ def main():
     import curvedspacesim_amd as csa
     from curvedspacesim_amd import workloads
 
+    if args.calibration_copy:
+        import torch
+
+        a = torch.ones(1 << 26, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("calibration copy: 268,435,456 bytes read and written")
+        del a, b
     w = workloads.config(args.config)
     log(f"workload: {w.name}")
     ctx = csa.Context(0)

@@ -300,7 +300,7 @@ csg_status runGeodesics(csg_ctx* c, csg::GeoParams& G, int maxTargets, bool whol
     int blocks = std::min(G.nSrc, 256);
     csg_status s = ensureScratch(c, c->nF, maxTargets, blocks);
     if (s) return s;
-    HIPCHECK(c, csg::launchGeodesicScratch(G, c->layout, nullptr, G.nSrc, blocks, c->stream));
+    HIPCHECK(c, csg::launchGeodesicScratchWide(G, c->layout, nullptr, G.nSrc, blocks, c->stream));
     std::vector<int> st(G.nSrc);
     HIPCHECK(c, hipMemcpyAsync(st.data(), G.status, sizeof(int) * G.nSrc, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));

@@ -30,7 +30,9 @@
 namespace csg {
 
 // device counters; CNT_LDS_REASON + b counts sources the LDS variant handed to the HBM variant for OverflowReason bit b
-enum Counter : int { CNT_CAND_OVERFLOW = 0, CNT_UNSOLVED = 1, CNT_WALK_FAIL = 2, CNT_PAIRS = 3, CNT_LDS_REASON = 4, CNT_KEPT = 10, CNT_N = 11 };
+enum Counter : int {
+    CNT_CAND_OVERFLOW = 0, CNT_UNSOLVED = 1, CNT_WALK_FAIL = 2, CNT_PAIRS = 3, CNT_LDS_REASON = 4, CNT_BAND = 10, CNT_KEPT = 11, CNT_N = 12
+};
 
 struct SystemState {
     int N = 0;
@@ -41,7 +43,7 @@ struct SystemState {
     DevBuf scratch;
     DevBuf scanTmp;  // tile sums and offsets of the tiled scan
     // learned screen: features and logits of the candidate pairs, kept-pair lists; recorded pair distances
-    DevBuf sfeat, scnt, soff, scand, sparams, pdist, skeep;
+    DevBuf sfeat, sparams, pdist, skeep, sband;
     // independent particle groups: group of each particle; per group potential, squared range, patch-radius cap,
     // truncation radius and sentinel distance
     int nGroups = 0;
@@ -112,7 +114,7 @@ void destroySystem(SystemState* s) {
     DevBuf* all[] = {&s->face, &s->bary, &s->vel, &s->force, &s->mass, &s->energy, &s->xyz, &s->sxyz, &s->key, &s->cellCount, &s->cellStart,
                      &s->cellFill, &s->order, &s->cnt, &s->off, &s->toff, &s->cand, &s->maxd, &s->rad, &s->disp, &s->tstatus,
                      &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->spill, &s->grab, &s->redo, &s->tspill, &s->scratch, &s->scratchBig,
-                     &s->sfeat, &s->scnt, &s->soff, &s->scand, &s->sparams, &s->pdist, &s->skeep,
+                     &s->sfeat, &s->sparams, &s->sband, &s->pdist, &s->skeep,
                      &s->grp, &s->gpot, &s->gr2, &s->gmaxd, &s->grmax, &s->gsent, &s->grange};
     for (DevBuf* b : all) b->release();
     for (auto& m : s->marks) {
@@ -505,16 +507,6 @@ __global__ void kScaleVel(int N, double* v, const double* ke) {
 
 }  // namespace
 
-// the kept candidates of each source, in their original order
-__global__ void kScreenScatter(int N, const int* toff, const int* cand, const uint8_t* keep, const int* soff, int* scand, int* counters) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    int o = soff[i];
-    for (int q = toff[i]; q < toff[i + 1]; ++q)
-        if (keep[q]) scand[o++] = cand[q];
-    if (i == 0) counters[CNT_KEPT] = soff[N];
-}
-
 // ---- step pieces (all asynchronous on the context stream) ------------------------------------------------------
 
 static csg_status sysNeighbors(csg_ctx* c, SystemState* S) {
@@ -591,18 +583,15 @@ static void screenFeatures(csg_ctx* c, SystemState* S) {
     (void)launchScreenFeatures(featureParams(c, S), (int)std::min<long>(S->candCap, 0x7fffffff), c->stream);
 }
 
-// screen the candidate pairs and point the solver at the kept lists
+// screen the candidate pairs: a keep flag per pair, which the solver reads (patches are still built from every
+// candidate), and the kept count
 static csg_status sysScreen(csg_ctx* c, SystemState* S, GeoParams& G) {
-    const int N = S->N;
     hipStream_t st = c->stream;
-    HIPCHECK(c, launchScreenSources(featureParams(c, S), S->sparams.as<float>(), S->screenThreshold, S->skeep.as<uint8_t>(), S->scnt.as<int>(),
-                                    st));
-    scanExclusive(st, S->scnt.as<int>(), S->soff.as<int>(), N, S->scanTmp.as<long long>());
-    hipLaunchKernelGGL(kScreenScatter, dim3(blocksFor(N)), dim3(kBlock), 0, st, N, S->toff.as<int>(), S->cand.as<int>(), S->skeep.as<uint8_t>(),
-                       S->soff.as<int>(), S->scand.as<int>(), S->counters.as<int>());
-    HIPCHECK(c, hipGetLastError());
-    G.toff = S->soff.as<int32_t>();
-    G.tidx = S->scand.as<int32_t>();
+    // counters CNT_BAND (the band list's length) and CNT_KEPT, adjacent, start at zero
+    hipLaunchKernelGGL(kFillInt, dim3(1), dim3(kBlock), 0, st, S->counters.as<int>() + CNT_BAND, 0, 2L);
+    HIPCHECK(c, launchScreenSources(featureParams(c, S), S->sparams.as<float>(), S->screenThreshold, S->skeep.as<uint8_t>(),
+                                    S->sband.as<int2>(), S->counters.as<int>() + CNT_BAND, st));
+    G.keep = S->skeep.as<uint8_t>();
     return CSG_OK;
 }
 
@@ -644,10 +633,7 @@ static GeoParams forceParams(csg_ctx* c, SystemState* S) {
         G.spill = S->spill.as<SpillRecord>();
         G.spillCap = kSpillCap;
     }
-    if (S->screenOn && c->submeshing) {
-        G.toff = S->soff.as<int32_t>();
-        G.tidx = S->scand.as<int32_t>();
-    }
+    if (S->screenOn && c->submeshing) G.keep = S->skeep.as<uint8_t>();
     return G;
 }
 
@@ -655,14 +641,13 @@ static csg_status sysForces(csg_ctx* c, SystemState* S) {
     csg_status s = sysNeighbors(c, S);
     if (s) return s;
     GeoParams G = forceParams(c, S);
-    G.toff = S->toff.as<int32_t>();  // the screen points the solver at its kept lists once they exist
-    G.tidx = S->cand.as<int32_t>();
+    G.keep = nullptr;  // set by the screen once this step's flags exist
     if (c->submeshing) {
         {
             MarkScope mark(S, c->stream, MARK_PATCH);
             HIPCHECK(c, launchPatch(G, S->plist.as<int32_t>(), S->pcount.as<int32_t>(), kPatchListCap, c->stream));
         }
-        if (S->screenOn) {  // the solver takes only the pairs the screen keeps; patches keep every candidate's face
+        if (S->screenOn) {  // the solver solves only the pairs the screen keeps; patches keep every candidate's face
             MarkScope mark(S, c->stream, MARK_SCREEN);
             csg_status s2 = sysScreen(c, S, G);
             if (s2) return s2;
@@ -686,7 +671,7 @@ static csg_status sysForces(csg_ctx* c, SystemState* S) {
         if (S->bigBlocks) HIPCHECK(c, launchGeodesicScratchRedo(G, S->layoutBig, S->bigBlocks, c->stream));
     } else {
         MarkScope mark(S, c->stream, MARK_GEODESIC_HBM);
-        HIPCHECK(c, launchGeodesicScratch(G, S->layout, nullptr, S->N, S->scratchBlocks, c->stream));
+        HIPCHECK(c, launchGeodesicScratchWide(G, S->layout, nullptr, S->N, S->scratchBlocks, c->stream));
     }
     hipLaunchKernelGGL(kCountNonzero, dim3(blocksFor(S->N)), dim3(kBlock), 0, c->stream, S->N, S->gstatus.as<int>(), S->counters.as<int>(),
                        (int)CNT_UNSOLVED, false);
@@ -1419,9 +1404,7 @@ csg_status csg_system_set_screen(csg_ctx* c, const float* params, float threshol
     }
     const size_t cap = (size_t)std::max<long>(S->candCap, 1);
     HIPCHECK(c, S->skeep.ensure(cap));
-    HIPCHECK(c, S->scand.ensure(4 * cap));
-    HIPCHECK(c, S->scnt.ensure(4 * ((size_t)S->N + 1)));
-    HIPCHECK(c, S->soff.ensure(4 * ((size_t)S->N + 1)));
+    HIPCHECK(c, S->sband.ensure(8 * cap));
     HIPCHECK(c, S->sparams.ensure(sizeof(float) * kScreenParams));
     HIPCHECK(c, hipMemcpyAsync(S->sparams.p, params, sizeof(float) * kScreenParams, hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));

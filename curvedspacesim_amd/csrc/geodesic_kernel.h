@@ -45,6 +45,7 @@ struct GeoParams {
     const int32_t* __restrict__ srcIdx;  // particle index of each source (nullptr: source i is particle i)
     const int32_t* __restrict__ toff;    // nSrc + 1 offsets into tidx and into the pair outputs
     const int32_t* __restrict__ tidx;    // particle index of each target
+    const uint8_t* __restrict__ keep;    // optional, per pair: 0 = not solved (the learned screen dropped it)
     const double* __restrict__ patchRadius;  // per source; nullptr: whole mesh (no submeshing)
     const int32_t* __restrict__ blockOrder;  // optional: source order (spatially sorted), split over the XCDs in chunks
     int blockChunk;                          // ceil(nSrc / kXcds) when blockOrder is set
@@ -108,6 +109,10 @@ hipError_t launchGeodesicLdsLarge(const GeoParams& P, int tier, int32_t* list, i
 hipError_t launchChildFrames(const DeviceMesh& mesh, double* out, hipStream_t stream);
 hipError_t launchGeodesicScratch(const GeoParams& P, const ScratchLayout& L, const int32_t* list, int nList, int blocks,
                                  hipStream_t stream);
+// the same with kWideLanes lanes per source, for whole-mesh solves
+constexpr int kWideLanes = 512;
+hipError_t launchGeodesicScratchWide(const GeoParams& P, const ScratchLayout& L, const int32_t* list, int nList, int blocks,
+                                     hipStream_t stream);
 // scratch variant over every source whose P.status is non-zero (left unsolved by the LDS variant); no host round trip
 hipError_t launchGeodesicScratchRedo(const GeoParams& P, const ScratchLayout& L, int blocks, hipStream_t stream);
 

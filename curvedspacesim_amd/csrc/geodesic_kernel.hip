@@ -863,7 +863,7 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int* cur
         for (int k = 0; k < 3; ++k) T.vfl[atomicAdd(&T.vfo[T.fcv[3 * s + k]], 1)] = s * 4 + k;
     for (int t = lane; t < nT; t += blockDim.x) {
         const int s = T.tslot[t];
-        if (s >= 0) T.tnext[t] = atomicExch(&T.thead[s], t);
+        if (s >= 0 && (!P.keep || P.keep[tb + t])) T.tnext[t] = atomicExch(&T.thead[s], t);  // screened-out pairs stay unlinked
     }
     __syncthreads();
     for (int s = lane; s < nF; s += blockDim.x) T.fhead[s] = T.thead[s];
@@ -1114,6 +1114,7 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
             if (P.pairFlags) P.pairFlags[out] = PAIR_OVERFLOW;
             continue;
         }
+        if (P.keep && !P.keep[out]) continue;  // a pair the screen dropped: not solved, no force
         const int f = P.pface[pj];
         const double* b = P.pbary + 3 * pj;
         double sb = b[0] + b[1] + b[2];
@@ -1351,8 +1352,11 @@ __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P, const i
 }
 
 // Scratch variant: the same solver with its tables in a per-workgroup slice of an HBM buffer, persistent over a
-// list of sources (sources the LDS variant could not hold, or whole-mesh solves).
-__global__ __launch_bounds__(64) void geodesicForceScratch(GeoParams P, ScratchLayout L, const int32_t* list, int nList, bool redoOnly) {
+// list of sources (sources the LDS variant could not hold, or whole-mesh solves). NT lanes per source: 64 for the
+// patches the LDS tables could not hold (many workgroups, few sources each), kWideLanes for whole-mesh solves, whose
+// fronts of thousands of windows fill them (few sources, one each).
+template <int NT>
+__global__ __launch_bounds__(NT) void geodesicForceScratch(GeoParams P, ScratchLayout L, const int32_t* list, int nList, bool redoOnly) {
     __shared__ Shared sh;
     char* base = (char*)L.base + (size_t)blockIdx.x * L.bytesPerBlock;
     size_t off = 0;
@@ -1566,13 +1570,20 @@ hipError_t launchGeodesicLds(const GeoParams& P, int variant, hipStream_t stream
 hipError_t launchGeodesicScratch(const GeoParams& P, const ScratchLayout& L, const int32_t* list, int nList, int blocks,
                                  hipStream_t stream) {
     if (nList <= 0) return hipSuccess;
-    hipLaunchKernelGGL(geodesicForceScratch, dim3(blocks), dim3(64), 0, stream, P, L, list, nList, false);
+    hipLaunchKernelGGL(geodesicForceScratch<64>, dim3(blocks), dim3(64), 0, stream, P, L, list, nList, false);
+    return hipGetLastError();
+}
+
+hipError_t launchGeodesicScratchWide(const GeoParams& P, const ScratchLayout& L, const int32_t* list, int nList, int blocks,
+                                     hipStream_t stream) {
+    if (nList <= 0) return hipSuccess;
+    hipLaunchKernelGGL(geodesicForceScratch<kWideLanes>, dim3(blocks), dim3(kWideLanes), 0, stream, P, L, list, nList, false);
     return hipGetLastError();
 }
 
 hipError_t launchGeodesicScratchRedo(const GeoParams& P, const ScratchLayout& L, int blocks, hipStream_t stream) {
     if (P.nSrc <= 0) return hipSuccess;
-    hipLaunchKernelGGL(geodesicForceScratch, dim3(blocks), dim3(64), 0, stream, P, L, (const int32_t*)nullptr, P.nSrc, true);
+    hipLaunchKernelGGL(geodesicForceScratch<64>, dim3(blocks), dim3(64), 0, stream, P, L, (const int32_t*)nullptr, P.nSrc, true);
     return hipGetLastError();
 }
 

@@ -420,58 +420,61 @@ __global__ void screenFeaturesKernel(ScreenFeatureParams P, int nPairs) {
     for (int k = 0; k < 4; ++k) out[k] = float4{o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]};
 }
 
-// Screening of the step, one source per wavefront pass (persistent wavefronts): every candidate closer than the guard
-// distance is kept without scoring; the others (the guard band, a few per source) get their features and their logit,
-// 16 to a tile, and are kept when it reaches the threshold. Writes the keep flag of every candidate pair and the kept
-// count of every source. A wavefront's band pairs pass their features to the lanes of the B operand through 1 KB of
-// wavefront-private LDS; the only workgroup barrier is the one after the weights are staged.
-__global__ __launch_bounds__(kScreenThreads) void screenSourcesKernel(ScreenFeatureParams P, const float* params, float threshold,
-                                                                     uint8_t* keep, int32_t* count) {
+// Screening of a force computation in two launches. The band pass (a wavefront per source, light enough for high
+// occupancy) keeps every candidate closer than the guard distance and appends the others, the guard band (a few per
+// source), to a compact list. The scoring pass (persistent wavefronts with the weights in registers) takes that list
+// 16 pairs to a tile: the lanes of the B operand compute their pair's features themselves, the MLP scores the tile,
+// and a band pair is kept when its logit reaches the threshold.
+__global__ __launch_bounds__(256) void screenBandKernel(ScreenFeatureParams P, uint8_t* keep, int2* band, int* bandCount, int* keptTotal) {
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int l = threadIdx.x & 63;
+    if (i >= P.nSrc) return;
+    const double rc = sourceCutoff(P, i);
+    const int b0 = P.toff[i], b1 = P.toff[i + 1];
+    const unsigned long long below = (1ull << l) - 1ull;
+    int kept = 0;
+    for (int c0 = b0; c0 < b1; c0 += 64) {
+        const int q = c0 + l;
+        const bool valid = q < b1;
+        const bool inBand = valid && cutoffMargin(P, i, P.tidx[q], rc) <= kScreenGuard;
+        if (valid) keep[q] = inBand ? 0 : 1;
+        const unsigned long long m = __ballot(inBand);
+        int base = 0;
+        if (l == 0 && m) base = atomicAdd(bandCount, __popcll(m));
+        base = __shfl(base, 0, 64);
+        if (inBand) band[base + __popcll(m & below)] = make_int2(q, i);
+        kept += __popcll(__ballot(valid && !inBand));
+    }
+    if (l == 0) atomicAdd(keptTotal, kept);
+}
+
+__global__ __launch_bounds__(kScreenThreads) void screenScoreKernel(ScreenFeatureParams P, const float* params, float threshold,
+                                                                   const int2* band, const int* bandCount, uint8_t* keep, int* keptTotal) {
     __shared__ Staged s;
-    __shared__ float rows[kScreenThreads / 64][16 * kScreenIn];
     stageWeights(params, s);
     __syncthreads();
     Frags F;
     loadFrags(s, F);
-    const int l = threadIdx.x & 63, r = l & 15, g = l >> 4, w = threadIdx.x >> 6;
-    float* xs = rows[w];
-    const unsigned long long below = (1ull << l) - 1ull;
+    const int l = threadIdx.x & 63, r = l & 15, g = l >> 4;
+    const int n = *bandCount;
     const int waves = gridDim.x * (kScreenThreads / 64);
-    for (int i = blockIdx.x * (kScreenThreads / 64) + w; i < P.nSrc; i += waves) {
-        const double rc = sourceCutoff(P, i);
-        const int b0 = P.toff[i], b1 = P.toff[i + 1];
-        int kept = 0;
-        for (int c0 = b0; c0 < b1; c0 += 64) {
-            const int q = c0 + l;
-            const bool valid = q < b1;
-            const int j = valid ? P.tidx[q] : 0;
-            const bool band = valid && cutoffMargin(P, i, j, rc) <= kScreenGuard;
-            bool keepIt = valid;
-            unsigned long long pending = __ballot(band);
-            while (pending) {
-                const int rank = __popcll(pending & below);
-                const bool mine = ((pending >> l) & 1ull) && rank < 16;
-                const int nRows = min(16, __popcll(pending));
-                if (mine) {
-                    float o[kScreenIn];
-                    pairFeatures(P, i, j, rc, o);
-                    for (int k = 0; k < kScreenIn; ++k) xs[rank * kScreenIn + k] = o[k];
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                bf16x8 x;
-                for (int k = 0; k < 8; ++k) x[k] = (g < 2 && r < nRows) ? bf(xs[r * kScreenIn + 8 * g + k]) : bfBits(0);
-                const float z = mlpTile(F, x);
-                const float zr = __shfl(z, mine ? rank : 0, 64);  // lane `rank` holds the logit of row `rank`
-                if (mine) keepIt = zr >= threshold;
-                pending &= ~__ballot(mine);
-                __builtin_amdgcn_wave_barrier();  // this batch's rows are read before the next batch writes them
-            }
-            if (valid) keep[q] = keepIt ? 1 : 0;
-            kept += __popcll(__ballot(keepIt));
+    for (int t = blockIdx.x * (kScreenThreads / 64) + (threadIdx.x >> 6); 16 * t < n; t += waves) {
+        const int row = 16 * t + r;
+        const bool mine = row < n;
+        int2 e = make_int2(0, 0);
+        bf16x8 x;
+        for (int k = 0; k < 8; ++k) x[k] = bfBits(0);
+        if (mine) e = band[row];
+        if (mine && g < 2) {  // two lanes per pair, each computing the features and keeping its half
+            float o[kScreenIn];
+            pairFeatures(P, e.y, P.tidx[e.x], sourceCutoff(P, e.y), o);
+            for (int k = 0; k < 8; ++k) x[k] = bf(o[8 * g + k]);
         }
-        if (l == 0) count[i] = kept;
+        const float z = mlpTile(F, x);
+        const bool keepIt = mine && g == 0 && z >= threshold;
+        if (mine && g == 0) keep[e.x] = keepIt ? 1 : 0;
+        const int k = __popcll(__ballot(keepIt));
+        if (l == 0 && k) atomicAdd(keptTotal, k);
     }
 }
 
@@ -487,11 +490,12 @@ hipError_t launchScreenForward(const float* params, const float* X, int n, const
     return hipGetLastError();
 }
 
-hipError_t launchScreenSources(const ScreenFeatureParams& P, const float* params, float threshold, uint8_t* keep, int32_t* count,
-                               hipStream_t stream) {
+hipError_t launchScreenSources(const ScreenFeatureParams& P, const float* params, float threshold, uint8_t* keep, int2* band,
+                               int* counts, hipStream_t stream) {
     if (P.nSrc <= 0) return hipSuccess;
-    const int blocks = std::min<int>((P.nSrc + 3) / 4, kScreenPersistentBlocks);
-    hipLaunchKernelGGL(screenSourcesKernel, dim3(blocks), dim3(kScreenThreads), 0, stream, P, params, threshold, keep, count);
+    hipLaunchKernelGGL(screenBandKernel, dim3((P.nSrc + 3) / 4), dim3(256), 0, stream, P, keep, band, counts, counts + 1);
+    hipLaunchKernelGGL(screenScoreKernel, dim3(kScreenPersistentBlocks), dim3(kScreenThreads), 0, stream, P, params, threshold,
+                       (const int2*)band, (const int*)counts, keep, counts + 1);
     return hipGetLastError();
 }
 
