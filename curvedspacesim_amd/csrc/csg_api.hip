@@ -14,6 +14,57 @@
 #include "geodesic_kernel.h"
 #include "transport_kernel.h"
 
+// ---- csg_pair_forces_noalloc: its scratch is a static device array --------------------------------------------------
+namespace {
+
+constexpr int kNoAllocMaxSources = 2048;          // sources per call
+constexpr int kNoAllocSpill = 256;                // spill records per source (beyond: the HBM-table tier)
+constexpr int kNoAllocFallbackBlocks = 16;        // workgroups of the HBM-table tier
+constexpr long kNoAllocFallbackFaces = 1536;      // faces and targets per source the HBM tables hold
+constexpr long kNoAllocFallbackTargets = 4096;
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+
+}  // namespace
+
+// The scratch of csg_pair_forces_noalloc: one copy per GPU, held until the process ends. Host code uses the device
+// address that hipGetSymbolAddress gives for it (csg_ctx::noallocScratch), never this name.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+// byte offsets of the regions of the scratch array
+struct NoAllocLayout {
+    size_t status, pcount, counter, forces, energy, plist, spill, tables, total;
+    csg::ScratchLayout tablesLayout;
+};
+
+NoAllocLayout noAllocLayout() {
+    auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    NoAllocLayout o{};
+    size_t off = 0;
+    const size_t n = kNoAllocMaxSources;
+    o.status = off, off += r(4 * n);
+    o.pcount = off, off += r(4 * n);
+    o.counter = off, off += r(4);
+    o.forces = off, off += r(24 * n);
+    o.energy = off, off += r(8 * n);
+    o.plist = off, off += r(4 * n * csg::kPatchListCap);
+    o.spill = off, off += r(sizeof(csg::SpillRecord) * n * kNoAllocSpill);
+    o.tablesLayout = csg::makeScratchLayout(kNoAllocFallbackFaces, kNoAllocFallbackTargets);
+    o.tables = off, off += o.tablesLayout.bytesPerBlock * kNoAllocFallbackBlocks;
+    o.total = off;
+    return o;
+}
+
+__global__ void kNoAllocZero(int* p) { *p = 0; }
+
+__global__ void kNoAllocCountUnsolved(int n, const int* status, int* counter) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && status[i] != 0) atomicAdd(counter, 1);
+}
+
+}  // namespace
+
 namespace csg {
 
 long patchFaceBound(const csg_ctx* c, double radius) {
@@ -333,6 +384,15 @@ csg_status csg_ctx_create(csg_ctx** out, int device) {
         delete c;
         return CSG_ERR_DEVICE;
     }
+    // the device address of csg_pair_forces_noalloc's scratch on this GPU (this device is current)
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+        c->noallocErr = std::string("hipGetSymbolAddress: ") + hipGetErrorString(e);
+    else if (noAllocLayout().total > sizeof(scratch))
+        c->noallocErr = "the scratch layout exceeds the static array";
+    else
+        c->noallocScratch = p;
     *out = c;
     return CSG_OK;
 }
@@ -440,6 +500,70 @@ csg_status csg_pair_forces(csg_ctx* c, int32_t nP, const int32_t* face, const do
     HIPCHECK(c, download(c, forces, c->pforces, 3 * (size_t)nSrc));
     HIPCHECK(c, download(c, energy, c->penergy, (size_t)nSrc));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return CSG_OK;
+}
+
+csg_status csg_pair_forces_noalloc(csg_ctx* c, int32_t nP, const int32_t* face, const double* bary, int32_t nSrc, const int32_t* srcIdx,
+                                   const int32_t* tgtOffsets, const int32_t* tgtIdx, const double* patchRadius, double rMax,
+                                   const csg_potential* pot, int32_t useSentinel, double sentinelDistance, double* dist, float* startTan,
+                                   float* endTan, uint32_t* flags, double* forces, double* energy) {
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!c->haveMesh) return fail(c, CSG_ERR_STATE, "no mesh uploaded");
+    if (!c->noallocScratch) return fail(c, CSG_ERR_DEVICE, "the static scratch of csg_pair_forces_noalloc is unavailable: " + c->noallocErr);
+    if (nP < 0 || nSrc < 0) return fail(c, CSG_ERR_ARGUMENT, "negative particle or source count");
+    if (nSrc > kNoAllocMaxSources)
+        return fail(c, CSG_ERR_CAPACITY, "csg_pair_forces_noalloc takes at most " + std::to_string(kNoAllocMaxSources) +
+                                             " sources per call (the size of its static scratch); split the sources over several calls");
+    if (nSrc == 0) return CSG_OK;
+    if (!face || !bary || !tgtOffsets || !tgtIdx) return fail(c, CSG_ERR_ARGUMENT, "null device arrays");
+    if (!patchRadius) return fail(c, CSG_ERR_ARGUMENT, "csg_pair_forces_noalloc needs patch radii (whole-mesh solves need larger tables)");
+    HIPCHECK(c, hipSetDevice(c->device));
+    const NoAllocLayout o = noAllocLayout();
+    char* base = (char*)c->noallocScratch;
+    csg::GeoParams G{};
+    G.mesh = c->dm;
+    G.pface = face;
+    G.pbary = bary;
+    G.nSrc = nSrc;
+    G.srcIdx = srcIdx;
+    G.toff = tgtOffsets;
+    G.tidx = tgtIdx;
+    G.patchRadius = patchRadius;
+    G.pairDist = dist;
+    G.pairStart = startTan;
+    G.pairEnd = endTan;
+    G.pairFlags = flags;
+    G.forces = forces ? forces : (double*)(base + o.forces);
+    G.energy = energy ? energy : (double*)(base + o.energy);
+    G.status = (int*)(base + o.status);
+    int32_t* plist = (int32_t*)(base + o.plist);
+    int32_t* pcount = (int32_t*)(base + o.pcount);
+    G.patchList = plist;
+    G.patchCount = pcount;
+    G.patchStride = csg::kPatchListCap;
+    G.spill = (csg::SpillRecord*)(base + o.spill);
+    G.spillCap = kNoAllocSpill;
+    G.pot = pot ? csg::Potential{pot->kind, pot->p0, pot->p1, pot->p2} : csg::Potential{csg::POT_NONE, 0, 0, 0};
+    G.useSentinel = useSentinel;
+    G.sentinelDist = sentinelDistance;
+    G.rMax = csg::exactTruncation(G.pot, rMax, useSentinel != 0, sentinelDistance);
+    // the same kernels as csg_pair_forces: patches, the LDS tables, then HBM tables of a fixed size for what they left
+    HIPCHECK(c, csg::launchPatch(G, plist, pcount, G.patchStride, c->stream));
+    HIPCHECK(c, csg::launchGeodesicLds(G, 0, c->stream));
+    csg::ScratchLayout L = o.tablesLayout;
+    L.base = base + o.tables;
+    HIPCHECK(c, csg::launchGeodesicScratchRedo(G, L, kNoAllocFallbackBlocks, c->stream));
+    int* counter = (int*)(base + o.counter);
+    hipLaunchKernelGGL(kNoAllocZero, dim3(1), dim3(1), 0, c->stream, counter);
+    hipLaunchKernelGGL(kNoAllocCountUnsolved, dim3((nSrc + 255) / 256), dim3(256), 0, c->stream, nSrc, (const int*)G.status, counter);
+    HIPCHECK(c, hipGetLastError());
+    int unsolved = 0;
+    HIPCHECK(c, hipMemcpyAsync(&unsolved, counter, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (unsolved)
+        return fail(c, CSG_ERR_CAPACITY, std::to_string(unsolved) + " sources exceed the tables of csg_pair_forces_noalloc (" +
+                                             std::to_string(kNoAllocFallbackFaces) + " patch faces, " + std::to_string(kNoAllocFallbackTargets) +
+                                             " targets per source); their outputs are not valid (csg_pair_forces grows its tables)");
     return CSG_OK;
 }
 

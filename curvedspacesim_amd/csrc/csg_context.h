@@ -67,6 +67,10 @@ struct csg_ctx {
     int scratchBlocks = 0;
     csg::ScratchLayout layout{};
     csg::SystemState* sys = nullptr;
+    // device address of the static scratch array of csg_pair_forces_noalloc on this context's GPU (null when it could
+    // not be resolved; noallocErr then says why)
+    void* noallocScratch = nullptr;
+    std::string noallocErr;
     ~csg_ctx() {
         if (sys) csg::destroySystem(sys);
         DevBuf* all[] = {&vx, &fv, &fadj, &fgeo, &fnrm, &vflag, &vfoff, &vflist, &curv, &fchild, &fcorn, &fnbv, &sX, &sY, &sOut, &sPart, &sGrad, &sParams, &sM, &sV, &pface, &pbary, &srcIdx, &toff, &tidx, &prad,

@@ -89,6 +89,19 @@ csg_status csg_pair_forces(csg_ctx* ctx, int32_t nP, const int32_t* face, const 
                            const csg_potential* pot, int32_t useSentinel, double sentinelDistance, double* dist, float* startTan,
                            float* endTan, uint32_t* flags, double* forces, double* energy);
 
+/* --- csg_pair_forces for callers that cannot allocate (no reference counterpart) ----------------------------------
+ * For code under HIP graph capture, tight serving loops and harnesses that forbid allocation: the arguments of
+ * csg_pair_forces, but every array is a device pointer the caller owns (inputs and outputs), and the call allocates
+ * nothing. Its scratch (patch lists, per-source status, spill queue, HBM tables for large patches) is a static device
+ * array of 64 MiB that each GPU the library runs on holds until the process ends. Limits: at most 2048 sources per call
+ * (CSG_ERR_CAPACITY beyond, before any work), patch radii required (no whole-mesh solves), and sources whose patches
+ * exceed 1536 faces or 4096 targets make the call return CSG_ERR_CAPACITY after it waited for the stream (their outputs
+ * are then not valid). Synchronous. Calls that overlap on one GPU race on the shared scratch. */
+csg_status csg_pair_forces_noalloc(csg_ctx* ctx, int32_t nP, const int32_t* face, const double* bary, int32_t nSrc, const int32_t* srcIdx,
+                                   const int32_t* tgtOffsets, const int32_t* tgtIdx, const double* patchRadius, double rMax,
+                                   const csg_potential* pot, int32_t useSentinel, double sentinelDistance, double* dist, float* startTan,
+                                   float* endTan, uint32_t* flags, double* forces, double* energy);
+
 /* --- open meshes: the space class decides what a walk does at the mesh border ------------------------------------
  * CSG_BOUNDARY_CLOSED: triangulatedMeshSpace (a border edge or vertex is a walk error, triangulatedMeshSpace.cpp:522-548);
  * CSG_BOUNDARY_ABSORBING: absorbingOpenMeshSpace (src/models/absorbingOpenMeshSpace.cpp:2-50), the particle stops where
