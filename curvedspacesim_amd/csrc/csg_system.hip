@@ -43,7 +43,7 @@ struct SystemState {
     DevBuf scratch;
     DevBuf scanTmp;  // tile sums and offsets of the tiled scan
     // learned screen: features and logits of the candidate pairs, kept-pair lists; recorded pair distances
-    DevBuf sfeat, sparams, pdist, skeep, sband;
+    DevBuf sfeat, sparams, pdist, skeep;
     // independent particle groups: group of each particle; per group potential, squared range, patch-radius cap,
     // truncation radius and sentinel distance
     int nGroups = 0;
@@ -114,7 +114,7 @@ void destroySystem(SystemState* s) {
     DevBuf* all[] = {&s->face, &s->bary, &s->vel, &s->force, &s->mass, &s->energy, &s->xyz, &s->sxyz, &s->key, &s->cellCount, &s->cellStart,
                      &s->cellFill, &s->order, &s->cnt, &s->off, &s->toff, &s->cand, &s->maxd, &s->rad, &s->disp, &s->tstatus,
                      &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->spill, &s->grab, &s->redo, &s->tspill, &s->scratch, &s->scratchBig,
-                     &s->sfeat, &s->sparams, &s->sband, &s->pdist, &s->skeep,
+                     &s->sfeat, &s->sparams, &s->pdist, &s->skeep,
                      &s->grp, &s->gpot, &s->gr2, &s->gmaxd, &s->grmax, &s->gsent, &s->grange};
     for (DevBuf* b : all) b->release();
     for (auto& m : s->marks) {
@@ -587,10 +587,10 @@ static void screenFeatures(csg_ctx* c, SystemState* S) {
 // candidate), and the kept count
 static csg_status sysScreen(csg_ctx* c, SystemState* S, GeoParams& G) {
     hipStream_t st = c->stream;
-    // counters CNT_BAND (the band list's length) and CNT_KEPT, adjacent, start at zero
+    // counters CNT_BAND (pairs scored) and CNT_KEPT, adjacent, start at zero
     hipLaunchKernelGGL(kFillInt, dim3(1), dim3(kBlock), 0, st, S->counters.as<int>() + CNT_BAND, 0, 2L);
     HIPCHECK(c, launchScreenSources(featureParams(c, S), S->sparams.as<float>(), S->screenThreshold, S->skeep.as<uint8_t>(),
-                                    S->sband.as<int2>(), S->counters.as<int>() + CNT_BAND, st));
+                                    S->counters.as<int>() + CNT_BAND, st));
     G.keep = S->skeep.as<uint8_t>();
     return CSG_OK;
 }
@@ -1404,7 +1404,6 @@ csg_status csg_system_set_screen(csg_ctx* c, const float* params, float threshol
     }
     const size_t cap = (size_t)std::max<long>(S->candCap, 1);
     HIPCHECK(c, S->skeep.ensure(cap));
-    HIPCHECK(c, S->sband.ensure(8 * cap));
     HIPCHECK(c, S->sparams.ensure(sizeof(float) * kScreenParams));
     HIPCHECK(c, hipMemcpyAsync(S->sparams.p, params, sizeof(float) * kScreenParams, hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));

@@ -54,9 +54,9 @@ struct ScreenFeatureParams {
 };
 hipError_t launchScreenFeatures(const ScreenFeatureParams& P, int nPairs, hipStream_t stream);
 // the screening of a force computation: per candidate pair a keep flag (pairs closer than the guard distance are kept
-// unscored; the others are scored and kept when their logit reaches the threshold). band: room for every candidate
-// pair (the guard-band list); counts[0] (band pairs) and counts[1] (pairs kept) must be zero before the call.
-hipError_t launchScreenSources(const ScreenFeatureParams& P, const float* params, float threshold, uint8_t* keep, int2* band,
-                               int* counts, hipStream_t stream);
+// unscored; the others are scored and kept when their logit reaches the threshold). counts[0] (pairs scored, the guard
+// band) and counts[1] (pairs kept) must be zero before the call.
+hipError_t launchScreenSources(const ScreenFeatureParams& P, const float* params, float threshold, uint8_t* keep, int* counts,
+                               hipStream_t stream);
 
 }  // namespace csg
