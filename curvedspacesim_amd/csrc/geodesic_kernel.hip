@@ -13,8 +13,10 @@
 //                Vertex distance bounds are LDS 64-bit atomic minima on the IEEE bit pattern (distances are
 //                non-negative), and a window is dropped when those bounds show it cannot carry a shortest path
 //                anywhere on its interval (Xin & Wang's filter). Saddle vertices (angle sum >= 2 pi) and vertices on
-//                the patch border are pseudo-sources: reaching one with a smaller distance queues a vertex event
-//                that opens a window on the far edge of each incident patch face.
+//                the patch border are pseudo-sources: each opens a window on the far edge of each incident patch
+//                face. The first LDS tables mark an improved vertex pending and fan it out once no window of a batch
+//                can lower its distance much further (so that a vertex improved several times fans out about once);
+//                the larger tables queue a vertex event per improvement.
 //   3. targets : every window is tested against the targets of the face it enters; a target keeps the minimum
 //                (atomic on the bit pattern) and, in a second pass, the tangents of the path that set it. The start
 //                tangent is carried per window as a rotation from the window's edge frame to the source-face frame,
@@ -41,6 +43,14 @@ constexpr double kEdgeTol = 1e-12;    // relative distance at which an interval 
 constexpr int kMaxSteps = 1 << 18;    // FIFO steps per source before it is abandoned as over capacity
 constexpr int kMaxWaves = 16;         // wavefronts per workgroup
 constexpr int kVertexFace = 1 << 30;  // tag of the work item that opens a pseudo-source window across one face
+// Pseudo-source fan-out. Pending marks (the first LDS tables): a pending vertex fans out once its distance is at most
+// the batch's smallest key plus this fraction of the truncation radius; the margin trades windows opened from
+// distances that are superseded later (small margins wait longer and take more steps) against the steps themselves
+// (measured on the bumpy sphere L6, N = 20,000, cutoff 0.15: 0.1 is the fastest, 4% below vertex events; 0.2 and more
+// overflow the work queue). Vertex events (the larger LDS tables and the HBM tables, whose scan of the pending marks
+// would cover up to 900 vertex slots a step; measured 5% faster there): a work item per improvement, which fans out
+// when it comes up if the distance still stands.
+constexpr double kReleaseSlack = 0.1;
 
 __device__ inline unsigned long long dbits(double d) { return (unsigned long long)__double_as_longlong(d); }
 __device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
@@ -64,7 +74,8 @@ struct Tables {
     int32_t* vglob;  // per vertex slot: global vertex id (setup only)
     unsigned long long* vdist;
     float2* vtan;    // start tangent of the best path to the vertex (source-face frame)
-    uint8_t* vflg;   // 1: may act as a pseudo-source
+    uint8_t* vflg;   // bit 0: may act as a pseudo-source; bit 1 (with pending marks): pending fan-out, set and cleared
+                     // with 32-bit atomics on the flag's word (the array is 4-byte aligned, padded to a multiple of 4)
     int32_t* vfo;    // per vertex slot: end of its range in vfl (ranges are consecutive, the first starts at 0)
     IT* vfl;         // patch faces around each vertex: face slot * 4 + the vertex's corner index
     // FIFO ring of work items: windows and vertex events
@@ -88,6 +99,8 @@ struct Shared {
     int spillHead, spillTail;  // the source's HBM spill queue: records [spillHead, spillTail), ring positions mod spillCap
     int next;                  // the source the workgroup takes next (dynamic distribution)
     int srcSlot;   // face slot of the source face
+    unsigned long long minKey;  // bit pattern of the smallest window key of the current batch (all keys are >= 0)
+    int nPend;                  // pseudo-source vertices marked pending (flag bit 1) and not yet fanned out
     double S[3], t1[3], t2[3];
     int wsum[kMaxWaves];
 };
@@ -445,6 +458,16 @@ __device__ inline void pushVertexEvent(const Ctx<TT>& C, int vslot, double d) {
     pushItem(C, 0.0, 0.0, 0.0, 0.0, d, -vslot - 1, make_float2(0.f, 0.f));
 }
 
+// Marks a pseudo-source vertex whose distance improved as pending: its fan-out waits until no window of a batch can
+// lower the distance further (propagateAll). The bit is set once (the 32-bit atomic's old value), so nPend counts
+// each pending vertex once.
+template <class TT>
+__device__ inline void markPending(const TT& T, Shared& sh, int v) {
+    const unsigned bit = 2u << (8 * (v & 3));
+    const unsigned old = atomicOr(reinterpret_cast<unsigned*>(T.vflg) + (v >> 2), bit);
+    if (!(old & bit)) atomicAdd(&sh.nPend, 1);
+}
+
 template <class TT>
 __device__ inline void pushVertexFace(const Ctx<TT>& C, int gs, int j, double d, float2 tv) {
     pushItem(C, 0.0, 0.0, 0.0, 0.0, d, kVertexFace | (gs * 4 + j), tv);
@@ -581,9 +604,9 @@ __device__ void fanOutVertex(const Ctx<TT>& C, int vslot, double d) {
 }
 
 // Pass 2 for a lane's three vertex candidates (distinct corners of one face): a candidate whose offer still stands
-// sets the vertex's tangent and, for a pseudo-source vertex, pushes its event. The six LDS reads are issued together
-// (slots clamped to 0 for empty candidates), so they cost one round trip.
-template <class TT>
+// sets the vertex's tangent and, for a pseudo-source vertex, marks it pending or pushes its event. The six LDS reads
+// are issued together (slots clamped to 0 for empty candidates), so they cost one round trip.
+template <bool kPending, class TT>
 __device__ inline void finishVertexCands(const Ctx<TT>& C, const VCand& c0, const VCand& c1, const VCand& c2) {
     const TT& T = C.T;
     const int s0 = c0.slot < 0 ? 0 : c0.slot, s1 = c1.slot < 0 ? 0 : c1.slot, s2 = c2.slot < 0 ? 0 : c2.slot;
@@ -592,7 +615,12 @@ __device__ inline void finishVertexCands(const Ctx<TT>& C, const VCand& c0, cons
     auto finish = [&](const VCand& c, unsigned long long d, uint8_t f) {
         if (c.slot < 0 || !c.improved || d != dbits(c.d)) return;
         T.vtan[c.slot] = c.t;
-        if (f && c.d <= C.rMax) pushVertexEvent(C, c.slot, c.d);
+        if ((f & 1) && c.d <= C.rMax) {
+            if constexpr (kPending)
+                markPending(T, C.sh, c.slot);
+            else
+                pushVertexEvent(C, c.slot, c.d);
+        }
     };
     finish(c0, d0, f0);
     finish(c1, d1, f1);
@@ -897,8 +925,8 @@ __device__ inline SourceEdge loadSourceEdge(const DeviceMesh& M, int sf, int k) 
 // up the source frame
 __device__ inline int sourceEdgeOf(int lane) { return lane - (blockDim.x >= 128 ? 64 : 0); }
 
-template <class TT>
-__device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf, int tb, SpillRecord* spill, double rMax,
+template <bool kPending, class TT>
+__device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf, int tb, SpillRecord* spill, double rMax, double slack,
                              const SourceEdge& se) {
     const int lane = threadIdx.x;
     const int ek = sourceEdgeOf(lane);
@@ -942,7 +970,12 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
     if (lane < 3) {
         int vs = T.fcv[3 * ss + lane];
         double d = bitsd(T.vdist[vs]);
-        if (T.vflg[vs] && d <= C.rMax) pushVertexEvent(C, vs, d);
+        if ((T.vflg[vs] & 1) && d <= C.rMax) {
+            if constexpr (kPending)
+                markPending(T, sh, vs);
+            else
+                pushVertexEvent(C, vs, d);
+        }
     }
     __syncthreads();
     int h = 0;  // every lane advances its own copy of the head
@@ -951,8 +984,10 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
         const int s0 = sh.spillHead;
         const int sp = sh.spillTail - s0;  // spilled records waiting
         const bool ovf = sh.overflow != 0;
+        const int np = sh.nPend;
+        if (lane == 0) sh.minKey = ~0ull;  // last read in the previous step's first pass
         ldsBarrier();  // every wave has read the tail and the flags before any wave appends or sets one
-        if (ovf || (h >= t && sp == 0)) break;
+        if (ovf || (h >= t && sp == 0 && np == 0)) break;
         if (step >= kMaxSteps) {  // every source ends: an unfinished one is reported like a capacity overflow
             if (lane == 0) atomicOr(&sh.overflow, OVF_STEPS);
             break;
@@ -994,14 +1029,14 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
                 fs = (fk & (kVertexFace - 1)) >> 2;
                 k = (fk & 3) + 1;  // the far edge of the face, seen from the vertex at corner (fk & 3)
                 if (k == 3) k = 0;
-            } else if (fk >= 0) {
+            } else if (kPending || fk >= 0) {  // (no vertex events with pending marks)
                 kind = 0;
                 fs = fk >> 3;
                 k = (fk >> 1) & 3;
             } else {
                 kind = 1;
             }
-            if (kind != 1) {  // geometry of the edge the item works on, fetched before the barrier
+            if (kPending || kind != 1) {  // geometry of the edge the item works on, fetched before the barrier
                 g = T.fglob[fs];
                 const double* geo = M.fgeo + 9 * g + 3 * k;
                 L = geo[0];
@@ -1024,6 +1059,17 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
                 K.dC = bitsd(T.vdist[K.vC]);
             }
         }
+        // lower bounds of the interval-end distances: a window near the pruning thresholds is propagated rather than
+        // dropped (propagating a useless window changes no distance); the same values scale the target tests'
+        // tolerance in both passes. With pending marks they are needed here, for the batch's smallest key (a lower
+        // bound of every distance its items and their descendants set); otherwise they are computed in the first
+        // pass (measured faster for each).
+        double r0 = 0, r1 = 0;
+        if constexpr (kPending) {
+            r0 = endDistLow(b0, sx, sy);
+            r1 = endDistLow(b1, sx, sy);
+            if (mine) atomicMin(&sh.minKey, dbits(kind == 0 ? keyOf(b0, b1, sx, sy, sg, r0, r1) : sg));
+        }
         // once every lane holds its record, the ring slots [h, h + nRing) are free for this step's appends
         ldsBarrier();
         C.qBase = h + nRing;
@@ -1031,13 +1077,13 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
         bool live = false;
         if (mine) {
             if (kind == 0) {
-                // lower bounds of the interval-end distances: a window near the pruning thresholds is propagated
-                // rather than dropped (propagating a useless window changes no distance); the same values scale the
-                // target tests' tolerance in both passes
-                const double r0 = endDistLow(b0, sx, sy), r1 = endDistLow(b1, sx, sy);
+                if constexpr (!kPending) {
+                    r0 = endDistLow(b0, sx, sy);
+                    r1 = endDistLow(b1, sx, sy);
+                }
                 live = keyOf(b0, b1, sx, sy, sg, r0, r1) <= C.rMax && !filteredBy(K.dA, K.dB, L, b0, b1, sg, r0, r1);
                 if (live) propagate(C, fk, g, L, x2, y2, b0, b1, sx, sy, sg, r0, r1, cf, rot, K, cP0, cP1, cP2);
-            } else if (kind == 1) {
+            } else if (!kPending && kind == 1) {
                 int vs = -fk - 1;
                 if (T.vdist[vs] == dbits(sg) && sg <= C.rMax) fanOutVertex(C, vs, sg);
             } else {
@@ -1050,12 +1096,27 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
                 }
             }
         }
+        if (kPending && np > 0 && lane >= (int)blockDim.x - 64) {
+            // pending pseudo-source vertices, on the last wavefront (batches fill the lanes from the first): a vertex
+            // fans out once no window of this batch can lower its distance (every distance they set is at least the
+            // batch's smallest key; with no window in the batch, every pending vertex goes). A later improvement
+            // marks it again; the items of the superseded fan-out are then dropped when they come up.
+            const unsigned long long mk = sh.minKey;
+            for (int v = lane - ((int)blockDim.x - 64); v < sh.nVerts; v += 64) {
+                if (!(T.vflg[v] & 2)) continue;
+                const unsigned long long db = T.vdist[v];
+                // (the bit patterns of non-negative doubles order as their values; mk = ~0: no item in the batch)
+                if (db > mk && bitsd(db) > bitsd(mk) + slack) continue;
+                atomicAnd(reinterpret_cast<unsigned*>(T.vflg) + (v >> 2), ~(2u << (8 * (v & 3))));
+                atomicSub(&sh.nPend, 1);
+                fanOutVertex(C, v, bitsd(db));
+            }
+        }
         ldsBarrier();
         if (live) {
             if (kind == 0 && T.fhead[fs] >= 0)
-                windowTargets<true, TT>(C, fs, g, k, L, x2, y2, b0, b1, sx, sy, sg, endDistLow(b0, sx, sy), endDistLow(b1, sx, sy), fk & 1,
-                                        rot);
-            finishVertexCands(C, cP0, cP1, cP2);
+                windowTargets<true, TT>(C, fs, g, k, L, x2, y2, b0, b1, sx, sy, sg, r0, r1, fk & 1, rot);
+            finishVertexCands<kPending>(C, cP0, cP1, cP2);
         }
         __syncthreads();  // a full barrier: this step's spill stores are visible to the next batches
         h += nRing;
@@ -1067,7 +1128,7 @@ __device__ void propagateAll(const GeoParams& P, const TT& T, Shared& sh, int sf
 }
 
 // kIt: faces per lane of a prepared patch list (0: no register caching, for tables of any size)
-template <int kIt, class TT>
+template <int kIt, bool kPending, class TT>
 __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src, int* cur, int* nxt, int frontCap) {
     const int lane = threadIdx.x;
     const DeviceMesh& M = P.mesh;
@@ -1083,7 +1144,7 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
     for (int i = lane; i <= T.vhmask; i += blockDim.x) T.vhkey[i] = -1;
     for (int i = lane; i < T.vcap; i += blockDim.x) T.vfo[i] = 0;
     if (lane == 0) {
-        sh.head = sh.tail = sh.nFaces = sh.nVerts = sh.nSteps = sh.nItems = sh.spillHead = sh.spillTail = 0;
+        sh.head = sh.tail = sh.nFaces = sh.nVerts = sh.nSteps = sh.nItems = sh.spillHead = sh.spillTail = sh.nPend = 0;
         sh.srcSlot = -1;
         sh.overflow = nT > T.tcap ? OVF_TARGETS : 0;
         sh.nCur = sh.nNext = sh.goals = 0;
@@ -1101,7 +1162,7 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
     const long long c0 = wall_clock64();
     if (ok) ok = buildPatch<kIt>(P, T, sh, cur, nxt, frontCap, src, sf, tb, nT);
     const long long c1 = wall_clock64();
-    if (ok) propagateAll(P, T, sh, sf, tb, spill, rMax, se);
+    if (ok) propagateAll<kPending>(P, T, sh, sf, tb, spill, rMax, kReleaseSlack * rMax, se);
     __syncthreads();
     const long long c2 = wall_clock64();
     // ---- finalize: paths through the target face's corners, direct visibility, outputs, force
@@ -1282,13 +1343,14 @@ __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P, const i
     static_assert(QC >= 4 * kMaxWaves, "the epilogue's reduction uses the ring's storage");
     static_assert(QC % 2 == 0 && (HF & (HF - 1)) == 0 && (HV & (HV - 1)) == 0, "alignment and power-of-two hashes");
     static_assert(4 * FC <= 32767 && VC <= 32767 && TC <= 32767, "slots must fit int16");
+    static_assert(VC % 4 == 0, "the vertex flags are updated as 32-bit words");
     static_assert(NT % 64 == 0 && NT / 64 <= kMaxWaves, "whole wavefronts");
     __shared__ __attribute__((aligned(16))) unsigned char ring[kRingBytes];
     __shared__ int32_t fglob[FC];
     __shared__ int16_t fcv[3 * FC], fnb[3 * FC], fhead[FC], vfl[3 * FC];
     __shared__ unsigned long long vdist[VC];
     __shared__ float2 vtan[VC];
-    __shared__ uint8_t vflg[VC];
+    __shared__ __attribute__((aligned(4))) uint8_t vflg[VC];
     __shared__ int32_t vfo[VC];
     __shared__ int16_t tnext[TC], tslot[TC];
     __shared__ double tq[3 * TC];
@@ -1335,7 +1397,7 @@ __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P, const i
     if constexpr (kList) {  // persistent over a device list of sources (the larger tables' pass)
         const int n = *count;
         for (int i = blockIdx.x; i < n; i += gridDim.x)
-            solveSource<(FC + NT - 1) / NT>(P, T, sh, list[i], FR > 0 ? front : nullptr, FR > 0 ? front + FR : nullptr, FR);
+            solveSource<(FC + NT - 1) / NT, false>(P, T, sh, list[i], FR > 0 ? front : nullptr, FR > 0 ? front + FR : nullptr, FR);
         return;
     }
     int src;
@@ -1348,7 +1410,7 @@ __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P, const i
         src = sourceOfBlock(P);
     }
     if (src < 0) return;
-    solveSource<(FC + NT - 1) / NT>(P, T, sh, src, nullptr, nullptr, 0);
+    solveSource<(FC + NT - 1) / NT, true>(P, T, sh, src, nullptr, nullptr, 0);
 }
 
 // Scratch variant: the same solver with its tables in a per-workgroup slice of an HBM buffer, persistent over a
@@ -1407,7 +1469,7 @@ __global__ __launch_bounds__(NT) void geodesicForceScratch(GeoParams P, ScratchL
     for (int i = blockIdx.x; i < nList; i += gridDim.x) {
         const int src = list ? list[i] : i;
         if (redoOnly && P.status[src] == 0) continue;  // solved by the LDS variant
-        solveSource<0>(Q, T, sh, src, cur, nxt, L.fcap);
+        solveSource<0, false>(Q, T, sh, src, cur, nxt, L.fcap);
     }
 }
 
