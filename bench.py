@@ -4,7 +4,7 @@ Workload (default): bumpy sphere L6 (81,920 faces), N = 20,000, truncated Gaussi
 Nose-Hoover chain NVT at T = 0.5, dt = 0.005; one step = 2 walks + 1 neighbour search + 1 batched exact-geodesic force
 evaluation, all resident in HBM. Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1|2|3] [--no-cpu-baseline]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1|2|3|5] [--no-cpu-baseline]
 """
 import argparse
 import dataclasses
@@ -36,6 +36,36 @@ def cpu_baseline(w, face, bary, vel, threads, budget_s=30.0, accuracy=True):
     if accuracy:
         log("accuracy of the GPU path against the restatement")
         r["accuracy_of_gpu_path"] = accuracy_check(w, face, bary, vel, threads)
+    return r
+
+
+def replicas_cpu_baseline(w, face, bary, vel, threads, budget_s=30.0):
+    """CPU baseline of a particle-group workload: each replica is one reference run on its own mesh, so one step of
+    all replicas costs the sum of the replicas' step times, each timed like a single system (_cpu_time) with an
+    equal share of the budget; at `threads` threads and, on a third of the budget, at one thread."""
+    reps = w.meta["replicas"]
+
+    def total(th, budget):
+        t, n_src, n_all = 0.0, 0, 0
+        for r, rep in enumerate(reps):
+            p0, n = rep["particle_offset"], rep["n"]
+            rw = dataclasses.replace(w, X=rep["X"], T=rep["T"], potential=w.meta["pots"][r], range=rep["sigma"],
+                                     maximum_distance=w.meta["maxd"][r], rmax=rep["sigma"], meta={})
+            one = _cpu_time(rw, face[p0:p0 + n] - rep["face_offset"], bary[p0:p0 + n], vel[p0:p0 + n], th, budget / len(reps))
+            t += 1.0 / one["value"]
+            n_src += one["sources_timed"]
+            n_all += n
+        return t, n_src, n_all
+
+    t, n_src, n_all = total(threads, budget_s)
+    r = dict(value=1.0 / t, unit="steps/s", cores=threads, kind="port",
+             sample=(f"sum over the {len(reps)} replicas of one step each (each replica one system on its own mesh): force "
+                     f"evaluation of {n_src} of {n_all} sources (reference semantics) scaled per replica, plus one walk "
+                     f"pass; oracle built -O3 -march=x86-64-v3 -ffast-math; {threads} OpenMP threads"))
+    if threads > 1:
+        t1, n1, _ = total(1, budget_s / 3)
+        r["single_thread"] = {"value": 1.0 / t1, "unit": "steps/s", "cores": 1,
+                              "sample": f"as above on one thread, {n1} of {n_all} sources"}
     return r
 
 
@@ -78,7 +108,7 @@ def _cpu_time(w, face, bary, vel, threads, budget_s):
                 sample=(f"force evaluation of {n_src} of {N} sources (reference semantics: whole patch, no truncation) "
                         f"scaled to {N}, plus {moves} full walk pass(es); oracle built -O3 -march=x86-64-v3 -ffast-math; "
                         f"{threads} OpenMP threads"),
-                pairs_per_s=float(off[-1]) / t_step)
+                pairs_per_s=float(off[-1]) / t_step, sources_timed=n_src)
 
 
 def profiled_traffic(config):
@@ -173,9 +203,9 @@ def screened_leg(ctx, w, sysm, steps, plain_steps_s, config):
             "train_and_calibrate_s": t_build, "kernel_ms_per_step": {k: val[0] / steps for k, val in kt.items()},
             "screen_ms_per_step": screen_ms, "pairs_scored_per_step": scored,
             "mfma": {"achieved": mlp_tflops, "peak": 2500.0, "unit": "TFLOP/s", "frac": mlp_tflops / 2500.0,
-                     "note": "MLP forward flops of the scored pairs over the whole screening launch group (candidate pass, "
-                             "band features and MLP, scan, compaction); the matrix-core rate of the MLP itself is that of "
-                             "the forward kernel over all candidates, in profiles/"}}
+                     "note": "MLP forward flops of the scored pairs over the whole screening launch (band test of every "
+                             "candidate, features and MLP of the band's pairs); the matrix-core rate of the MLP itself is "
+                             "that of the forward kernel over all candidates, in profiles/"}}
 
 
 def accuracy_check(w, face, bary, vel, threads):
@@ -246,10 +276,7 @@ def main():
               "work; sharding over several GPUs is designed but not built", file=sys.stderr)
         sys.exit(2)
 
-    import curvedspacesim_amd as csa
-    from curvedspacesim_amd import workloads
-
-    if args.calibration_copy:
+    if args.calibration_copy:  # before libcsg.so loads, so that both use the HIP runtime torch brings
         import torch
 
         a = torch.ones(1 << 26, dtype=torch.float32, device="cuda:0")
@@ -258,6 +285,9 @@ def main():
         torch.cuda.synchronize()
         log("calibration copy: 268,435,456 bytes read and written")
         del a, b
+
+    import curvedspacesim_amd as csa
+    from curvedspacesim_amd import workloads
     w = workloads.config(args.config)
     log(f"workload: {w.name}")
     ctx = csa.Context(0)
@@ -294,8 +324,13 @@ def main():
     # algorithmic bytes of one launch of the geodesic kernel: per source the patch's faces (3 corner ids + 3
     # neighbour ids, 24 B), its vertices (FP64 xyz, 24 B), its targets (index + face + FP64 barycentrics, 32 B), the
     # source's own state (28 B) and its force (24 B)
-    geo_ms, geo_n = kt["geodesic_lds"]
-    bytes_launch = 24.0 * ps["faces"] + 24.0 * ps["verts"] + 32.0 * pairs + 52.0 * len(w.face)
+    if w.submeshing:
+        geo_key, geo_kernel = "geodesic_lds", "geodesicForceLds"
+        bytes_launch = 24.0 * ps["faces"] + 24.0 * ps["verts"] + 32.0 * pairs + 52.0 * len(w.face)
+    else:  # whole-mesh solves (512-lane HBM-table kernel): every source reads the whole mesh
+        geo_key, geo_kernel = "geodesic_hbm", "geodesicForceScratch<512> (whole mesh)"
+        bytes_launch = len(w.face) * (24.0 * len(w.T) + 24.0 * len(w.X)) + 32.0 * pairs + 52.0 * len(w.face)
+    geo_ms, geo_n = kt[geo_key]
     avg_ms = geo_ms / max(geo_n, 1)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     out = {
@@ -328,7 +363,7 @@ def main():
         "lds_fallbacks_per_step": {k: v / (args.steps + args.warmup) for k, v in sysm.fallback_counts().items()},
         # the kernel is bound by the latency of its dependent LDS and memory round trips (the SQ counters of the committed
         # profile: waves waiting most of their cycles, VALU and FP64 issue far from peak); its HBM roofline is reported
-        "roofline": {"kernel": "geodesicForceLds", "bound": "latency", "peak_basis": "hbm", "achieved": achieved,
+        "roofline": {"kernel": geo_kernel, "bound": "latency", "peak_basis": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profiled_traffic(args.config), "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": bytes_launch, "counters": profiled_counters(args.config)},
@@ -336,8 +371,11 @@ def main():
     if not args.no_screen and args.config in (3, 5):
         out["screened"] = screened_leg(ctx, w, sysm, args.steps, steps_s, args.config)
     if not args.no_cpu_baseline and w.meta.get("groups") is not None:
-        log("no CPU baseline for particle groups: the restatement runs one system at a time")
-        out["cpu_baseline"] = None
+        f, b, v, _ = sysm.get()
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline with {threads} threads, replica by replica")
+        out["cpu_baseline"] = replicas_cpu_baseline(w, f, b, v, threads)
+        out["speedup_vs_cpu"] = steps_s / out["cpu_baseline"]["value"]
     elif not args.no_cpu_baseline:
         f, b, v, _ = sysm.get()
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
