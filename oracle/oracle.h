@@ -4,15 +4,22 @@
  * and bench.py's cpu_baseline leg load this library, and only as the checker; the product never links it.
  *
  * Parity status of each function (how it is pinned independently of this code):
+ *   orc_mesh_create     the reference's mesh checks: out-of-range indices, degenerate faces, non-manifold edges and
+ *                       inconsistent orientation are rejected
  *   orc_geodesic        brute-force unfolding on tiny meshes, closed forms (plane, cube, tetrahedron), symmetry,
  *                       triangle inequality, d >= Euclidean, full mesh = patch for short pairs
  *   orc_patch           hand-worked cases of the reference rule on a small strip mesh and invariants
- *   orc_transport       flat-mesh exact translation, cube unfolding, speed preservation, reversibility; open-mesh
- *                       border edges (absorbing stop point, tangential slide, vector projection) against closed forms on
- *                       flat meshes; the border-vertex rule (absorbing: stop at the vertex) likewise; the tangential
- *                       continuation after a border vertex only through invariants (parity unpinned there)
+ *   orc_transport       flat-mesh exact translation, cube unfolding, speed preservation, reversibility; interior vertex
+ *                       crossings worked by hand through the reference's steps (a flat grid and a cone apex, where the
+ *                       heading is rotated); open-mesh border edges (absorbing stop point, tangential slide, vector
+ *                       projection) against closed forms on flat meshes; a border vertex in both modes on exact dyadic
+ *                       inputs worked by hand, including the tangential continuation
  *   orc_cell_candidates brute-force O(N^2) Euclidean test and the reference cell order on hand-built grids
  *   orc_pair_force      closed forms of the potentials
+ *   orc_compute_forces, orc_move, with pyoracle.OracleSystem's integrators: a free particle moves on a straight line,
+ *                       a head-on collision on a flat strip matches its closed-form trajectory, a cylinder run equals
+ *                       its unrolled plane run, velocity Verlet's energy error falls as dt^2, and the Nose-Hoover
+ *                       chain's extended energy is conserved (chains of 2 and 3)
  * Functions that only a CGAL build could pin (CGAL's rounding and its choice among tied geodesics) are reported as
  * "parity unpinned" where the tests use them.
  */
