@@ -1394,10 +1394,18 @@ __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P, const i
     T.vcap = VC;
     T.qcap = QC;
     T.tcap = TC;
-    if constexpr (kList) {  // persistent over a device list of sources (the larger tables' pass)
-        const int n = *count;
-        for (int i = blockIdx.x; i < n; i += gridDim.x)
+    if constexpr (kList) {
+        // persistent over a device list of sources (the larger tables' pass): count[0] entries, handed out one at a
+        // time by the counter count[1] (zero at launch), so that a workgroup with a long source takes fewer
+        const int n = count[0];
+        for (;;) {
+            if (threadIdx.x == 0) sh.next = atomicAdd(const_cast<int32_t*>(count) + 1, 1);
+            __syncthreads();
+            const int i = sh.next;
+            __syncthreads();  // every lane has read sh.next before the source's setup writes the shared state
+            if (i >= n) break;
             solveSource<(FC + NT - 1) / NT, false>(P, T, sh, list[i], FR > 0 ? front : nullptr, FR > 0 ? front + FR : nullptr, FR);
+        }
         return;
     }
     int src;
