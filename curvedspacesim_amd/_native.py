@@ -4,6 +4,8 @@ Both libraries are built in-tree by curvedspacesim_amd/build.py. Loading fails l
 no Python or CPU fallback for anything in this package.
 """
 import ctypes as C
+import hashlib
+import json
 import os
 
 import numpy as np
@@ -217,6 +219,7 @@ class Context:
         tri = np.ascontiguousarray(tri, np.int32).reshape(-1, 3)
         self._check(self.L.csg_mesh_upload(self.h, len(xyz), xyz.ravel(), len(tri), tri.ravel()), "csg_mesh_upload")
         self.nV, self.nF = len(xyz), len(tri)
+        self._mesh_digest = hashlib.sha1(xyz.tobytes() + tri.tobytes()).hexdigest()[:16]
 
     def mesh_info(self):
         nv, nf, area = C.c_int32(), C.c_int32(), C.c_double()
@@ -236,6 +239,7 @@ class Context:
 
     def set_submeshing(self, enabled, maximum_distance=1.0):
         self._check(self.L.csg_set_submeshing(self.h, int(bool(enabled)), float(maximum_distance)), "csg_set_submeshing")
+        self._submeshing = (bool(enabled), float(maximum_distance))
 
     def distance_batch(self, src_face, src_bary, tgt_offsets, tgt_face, tgt_bary, threshold=None):
         src_face = np.ascontiguousarray(src_face, np.int32)
@@ -330,6 +334,7 @@ class System:
         rm = float(rmax) if np.isfinite(rmax) else 1e300
         self.ctx._check(self.L.csg_system_configure(self.ctx.h, C.byref(pot), float(range_), int(bool(cell_list)), rm,
                                                     int(max_patch_faces)), "csg_system_configure")
+        self._config = [[float(x) for x in potential], float(range_), bool(cell_list), rm, int(max_patch_faces)]
 
     def compute_forces(self):
         F = np.zeros((self.N, 3))
@@ -421,11 +426,30 @@ class System:
         names = ["neighbors", "geodesic_lds", "geodesic_hbm", "transport", "patch", "screen"]
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(names)}
 
-    def autotune(self):
-        """Times each workgroup size of the LDS geodesic kernel and keeps the fastest: (chosen, ms per variant)."""
+    def tuning_key(self):
+        """What the fastest LDS variant depends on: the device, the mesh (a digest of its arrays), the particle count,
+        the potential, range, cell list, truncation and patch bound, submeshing, and the particle groups."""
+        sub = getattr(self.ctx, "_submeshing", None)
+        groups = getattr(self, "_groups", None)
+        return json.dumps([self.ctx.device_name, getattr(self.ctx, "_mesh_digest", None), self.N, getattr(self, "_config", None),
+                           sub, groups], separators=(",", ":"))
+
+    def autotune(self, cache=None):
+        """Times each workgroup size of the LDS geodesic kernel (the variants of CSG_LDS_VARIANTS) and keeps the
+        fastest: (chosen, ms per force computation for each variant). With cache, the path of a JSON file: a system
+        whose tuning_key() the file holds takes the recorded choice without timing (and returns the recorded times);
+        otherwise the result of the timing is added to the file."""
+        key = self.tuning_key() if cache else None
+        if cache:
+            hit = lookup_tuning(cache, key)
+            if hit is not None:
+                self.set_lds_variant(hit[0])
+                return hit
         v = C.c_int32()
         ms = np.zeros(LDS_VARIANTS)
         self.ctx._check(self.L.csg_system_autotune(self.ctx.h, C.byref(v), ms), "csg_system_autotune")
+        if cache:
+            record_tuning(cache, key, v.value, ms)
         return v.value, ms
 
     def use_graph(self, enable=True):
@@ -439,9 +463,12 @@ class System:
         tuple, an interaction range and a maximum distance. group_of=None clears."""
         if group_of is None:
             self.ctx._check(self.L.csg_system_set_groups(self.ctx.h, 0, None, None, None, None), "csg_system_set_groups")
+            self._groups = None
             return
         g = np.ascontiguousarray(group_of, np.int32)
         n = len(potentials)
+        self._groups = [hashlib.sha1(g.tobytes()).hexdigest()[:16], [[float(x) for x in p] for p in potentials],
+                        [float(x) for x in ranges], [float(x) for x in maximum_distances]]
         pots = (csg_potential * n)(*[csg_potential(int(p[0]), *[float(x) for x in (list(p[1:]) + [0, 0, 0])[:3]])
                                      for p in potentials])
         r = np.ascontiguousarray(ranges, np.float64)
@@ -488,6 +515,38 @@ class System:
         return dict(faces=int(s[0]), verts=int(s[1]), steps=int(s[2]), items=int(s[3]), max_faces=int(m[0]),
                     max_steps=int(m[2]), max_items=int(m[3]), us_patch=s[4] * 0.01, us_propagate=s[5] * 0.01,
                     us_epilogue=s[6] * 0.01)
+
+
+def _read_tuning(path):
+    try:
+        with open(path) as fh:
+            table = json.load(fh)
+        return table if isinstance(table, dict) else {}
+    except (OSError, ValueError):
+        return {}
+
+
+def lookup_tuning(path, key):
+    """The recorded (variant, ms per variant) for key in the autotuner's JSON file, or None (no file, no entry, or an
+    entry that does not fit this library's variants)."""
+    e = _read_tuning(path).get(key)
+    try:
+        v, ms = int(e["variant"]), np.array(e["ms"], np.float64)
+    except (TypeError, KeyError, ValueError):
+        return None
+    if not 0 <= v < LDS_VARIANTS or len(ms) != LDS_VARIANTS or e.get("variants") != list(LDS_VARIANT_NAMES):
+        return None
+    return v, ms
+
+
+def record_tuning(path, key, variant, ms):
+    """Adds or replaces key's entry in the autotuner's JSON file (written to a temporary file, then renamed over it)."""
+    table = _read_tuning(path)
+    table[key] = {"variant": int(variant), "ms": [float(x) for x in ms], "variants": list(LDS_VARIANT_NAMES)}
+    tmp = f"{path}.{os.getpid()}.tmp"
+    with open(tmp, "w") as fh:
+        json.dump(table, fh, indent=1, sort_keys=True)
+    os.replace(tmp, path)
 
 
 CHECKPOINT_FIELDS = ("time", "R3position", "barycentricPosition", "faceIndex", "velocity", "force", "nhChain", "nhParams")

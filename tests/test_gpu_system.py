@@ -112,6 +112,28 @@ def test_lds_variants_and_autotune(ctx):
     assert sysm.status()["unsolved"] == 0
 
 
+def test_autotune_cache(ctx, tmp_path):
+    """The autotuner's JSON cache: a first system is timed and recorded; a second system of the same workload takes the
+    recorded choice without timing (the same times come back, and the same forces); another particle count is a
+    different key."""
+    cache = str(tmp_path / "tuning.json")
+    w = workloads.config(3, n=3000)
+    print("case autotune cache config 3 N=3000", flush=True)
+    sysm = workloads.make_system(ctx, w)
+    chosen, ms = sysm.autotune(cache=cache)
+    key = sysm.tuning_key()
+    assert csa._native.lookup_tuning(cache, key)[0] == chosen
+    F0, _ = sysm.compute_forces()
+    again = workloads.make_system(ctx, w)
+    assert again.tuning_key() == key
+    c2, ms2 = again.autotune(cache=cache)
+    assert c2 == chosen and np.array_equal(ms2, ms)
+    F1, _ = again.compute_forces()
+    assert np.max(np.abs(F1 - F0)) <= 1e-6 * np.abs(F0).max()  # tied windows' FP32 tangents may differ between runs
+    other = workloads.make_system(ctx, workloads.config(3, n=2000))
+    assert other.tuning_key() != key and csa._native.lookup_tuning(cache, other.tuning_key()) is None
+
+
 @pytest.mark.parametrize("integrator", ["nve", "nvt"])
 def test_graph_replays_match_eager(integrator):
     """Graph mode: one captured step replayed several times, with host reads and synchronisation between the replays,
