@@ -663,8 +663,8 @@ static csg_status sysForces(csg_ctx* c, SystemState* S) {
         MarkScope mark(S, c->stream, MARK_GEODESIC_HBM);
         // the sources the LDS variant could not hold: two tiers of larger LDS tables, then HBM tables for what remains
         for (int tier = 1; tier <= 2; ++tier) {
-            hipLaunchKernelGGL(kFillInt, dim3(1), dim3(kBlock), 0, c->stream, S->redo.as<int>(), 0, 2L);
-            HIPCHECK(c, launchGeodesicLdsLarge(G, tier, S->redo.as<int32_t>() + 2, S->redo.as<int32_t>(), S->tspill.as<SpillRecord>(),
+            hipLaunchKernelGGL(kFillInt, dim3(1), dim3(kBlock), 0, c->stream, S->redo.as<int>(), 0, (long)(1 + kTierBuckets));
+            HIPCHECK(c, launchGeodesicLdsLarge(G, tier, S->redo.as<int32_t>() + 1 + kTierBuckets, S->redo.as<int32_t>(), S->tspill.as<SpillRecord>(),
                                                c->stream));
         }
         HIPCHECK(c, launchGeodesicScratchRedo(G, S->layout, S->scratchBlocks, c->stream));
@@ -918,11 +918,12 @@ csg_status csg_system_init(csg_ctx* c, int32_t N, const int32_t* face, const dou
     HIPCHECK(c, S->tstatus.ensure(4 * (size_t)N));
     HIPCHECK(c, S->gstatus.ensure(4 * (size_t)N));
     HIPCHECK(c, S->pstat.ensure(4 * kPatchStats * (size_t)N));
+    HIPCHECK(c, hipMemsetAsync(S->pstat.p, 0, 4 * kPatchStats * (size_t)N, c->stream));  // no solve has a duration yet
     HIPCHECK(c, S->plist.ensure(4 * (size_t)kPatchListCap * N));
     HIPCHECK(c, S->pcount.ensure(4 * (size_t)N));
     HIPCHECK(c, S->spill.ensure(sizeof(SpillRecord) * (size_t)kSpillCap * N));
     HIPCHECK(c, S->grab.ensure(sizeof(int) * kXcds));
-    HIPCHECK(c, S->redo.ensure(sizeof(int) * ((size_t)N + 2)));
+    HIPCHECK(c, S->redo.ensure(sizeof(int) * ((size_t)kTierBuckets * N + 1 + kTierBuckets)));
     HIPCHECK(c, S->tspill.ensure(kTierSpillBytes));
     HIPCHECK(c, S->partial.ensure(8 * (size_t)blocksFor(N)));
     HIPCHECK(c, S->counters.ensure(4 * CNT_N));

@@ -101,9 +101,11 @@ constexpr int kLdsVariants = CSG_LDS_VARIANTS;
 hipError_t launchGeodesicLds(const GeoParams& P, int variant, hipStream_t stream);
 // the sources flagged for a table capacity (P.status), solved again with larger LDS tables, persistent over their
 // list: tier 1 up to 1,024 faces (two workgroups per CU), tier 2 up to 1,536 faces (one per CU, searching the patch
-// itself when the patch kernel could not list it). list: nSrc ints; count: two ints, both zero before the call (the
-// list's length, then the counter that hands its entries out). spill: kTierSpillBytes, shared by the workgroups
-// (P.spill is not used)
+// itself when the patch kernel could not list it), the longest first by their last solve's duration (P.patchStats,
+// when given). list: kTierBuckets * nSrc ints; count: 1 + kTierBuckets ints, zero before the call (the counter that
+// hands the entries out, then each bucket's length). spill: kTierSpillBytes, shared by the workgroups (P.spill is not
+// used)
+constexpr int kTierBuckets = 8;
 constexpr size_t kTierSpillBytes = (size_t)512 * 8192 * sizeof(SpillRecord);
 hipError_t launchGeodesicLdsLarge(const GeoParams& P, int tier, int32_t* list, int32_t* count, SpillRecord* spill, hipStream_t stream);
 // fills mesh.fchild (out, 18 per face) from mesh.fgeo

@@ -1287,7 +1287,9 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
         }
         if (P.energy) P.energy[src] = E;
         if (P.status) P.status[src] = sh.overflow;
-        if (P.patchStats) {  // counts, then phase durations in ticks of the constant 100 MHz clock
+        // counts, then phase durations in ticks of the constant 100 MHz clock; a source these tables could not hold
+        // keeps the figures of the tables that last solved it (the larger tiers order their work by that duration)
+        if (P.patchStats && !sh.overflow) {
             int* ps = P.patchStats + kPatchStats * src;
             ps[0] = sh.nFaces;
             ps[1] = sh.nVerts;
@@ -1395,16 +1397,27 @@ __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P, const i
     T.qcap = QC;
     T.tcap = TC;
     if constexpr (kList) {
-        // persistent over a device list of sources (the larger tables' pass): count[0] entries, handed out one at a
-        // time by the counter count[1] (zero at launch), so that a workgroup with a long source takes fewer
-        const int n = count[0];
+        // persistent over a device list of sources (the larger tables' pass), handed out one at a time by the counter
+        // count[0] (zero at launch): the longest first, bucket kTierBuckets - 1 down to bucket 0 (count[1 + b] entries
+        // at list + b * nSrc), so that no workgroup ends with a long source while the others idle
         for (;;) {
-            if (threadIdx.x == 0) sh.next = atomicAdd(const_cast<int32_t*>(count) + 1, 1);
+            if (threadIdx.x == 0) {
+                int i = atomicAdd(const_cast<int32_t*>(count), 1), s = -1;
+                for (int b = kTierBuckets - 1; b >= 0; --b) {
+                    const int nb = count[1 + b];
+                    if (i < nb) {
+                        s = list[(size_t)b * P.nSrc + i];
+                        break;
+                    }
+                    i -= nb;
+                }
+                sh.next = s;
+            }
             __syncthreads();
-            const int i = sh.next;
+            const int src = sh.next;
             __syncthreads();  // every lane has read sh.next before the source's setup writes the shared state
-            if (i >= n) break;
-            solveSource<(FC + NT - 1) / NT, false>(P, T, sh, list[i], FR > 0 ? front : nullptr, FR > 0 ? front + FR : nullptr, FR);
+            if (src < 0) break;
+            solveSource<(FC + NT - 1) / NT, false>(P, T, sh, src, FR > 0 ? front : nullptr, FR > 0 ? front + FR : nullptr, FR);
         }
         return;
     }
@@ -1583,10 +1596,19 @@ static void launchLdsVariant(const GeoParams& P, hipStream_t stream) {
 }
 
 // sources whose first solve overflowed a capacity of the LDS tables (not the step limit) into list[0 .. *count)
-__global__ void listOverflowedKernel(int n, const int* status, int32_t* list, int32_t* count) {
+// The sources flagged for a table capacity, by the duration of their last completed solve (patchStats, in ticks of
+// 100 MHz): bucket b holds durations of about 2^(b+10) ticks (10 us) and up, the last everything from 1.3 ms on.
+__global__ void listOverflowedKernel(int n, const int* status, const int* patchStats, int32_t* list, int32_t* count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (status[i] & (OVF_FACES | OVF_VERTS | OVF_QUEUE | OVF_TARGETS)) list[atomicAdd(count, 1)] = i;
+    if (!(status[i] & (OVF_FACES | OVF_VERTS | OVF_QUEUE | OVF_TARGETS))) return;
+    int b = 0;
+    if (patchStats) {
+        const int* ps = patchStats + (size_t)kPatchStats * i;
+        const unsigned t = (unsigned)max(ps[4], 0) + (unsigned)max(ps[5], 0) + (unsigned)max(ps[6], 0);
+        b = min(max((31 - __clz((int)(t | 1u))) - 10, 0), kTierBuckets - 1);
+    }
+    list[(size_t)b * n + atomicAdd(count + 1 + b, 1)] = i;
 }
 
 template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, int WGS, int FR>
@@ -1602,7 +1624,8 @@ static void launchLdsList(const GeoParams& P, const int32_t* list, const int32_t
 hipError_t launchGeodesicLdsLarge(const GeoParams& Q, int tier, int32_t* list, int32_t* count, SpillRecord* spill, hipStream_t stream) {
     if (Q.nSrc <= 0) return hipSuccess;
     GeoParams P = Q;
-    hipLaunchKernelGGL(listOverflowedKernel, dim3((P.nSrc + 255) / 256), dim3(256), 0, stream, P.nSrc, (const int*)P.status, list, count);
+    hipLaunchKernelGGL(listOverflowedKernel, dim3((P.nSrc + 255) / 256), dim3(256), 0, stream, P.nSrc, (const int*)P.status,
+                       (const int*)P.patchStats, list, count);
     // CUs of the current device (cached per device)
     static int cusOf[64] = {};
     int dev = 0;
