@@ -324,10 +324,10 @@ def main():
     # algorithmic bytes of one launch of the geodesic kernel: per source the patch's faces (3 corner ids + 3
     # neighbour ids, 24 B), its vertices (FP64 xyz, 24 B), its targets (index + face + FP64 barycentrics, 32 B), the
     # source's own state (28 B) and its force (24 B)
-    if w.submeshing:
+    if kt["geodesic_lds"][1] > 0:  # patches (also whole-mesh mode with a finite truncation radius)
         geo_key, geo_kernel = "geodesic_lds", "geodesicForceLds"
         bytes_launch = 24.0 * ps["faces"] + 24.0 * ps["verts"] + 32.0 * pairs + 52.0 * len(w.face)
-    else:  # whole-mesh solves (512-lane HBM-table kernel): every source reads the whole mesh
+    else:  # whole-mesh solves on HBM tables (512 lanes per source): every source reads the whole mesh
         geo_key, geo_kernel = "geodesic_hbm", "geodesicForceScratch<512> (whole mesh)"
         bytes_launch = len(w.face) * (24.0 * len(w.T) + 24.0 * len(w.X)) + 32.0 * pairs + 52.0 * len(w.face)
     geo_ms, geo_n = kt[geo_key]

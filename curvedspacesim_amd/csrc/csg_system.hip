@@ -66,6 +66,9 @@ struct SystemState {
     ScratchLayout layoutBig{};
     int bigBlocks = 0;
     double bigRadius = 0;
+    // whole-mesh mode with a finite truncation radius: the patch radius that holds every geodesic within it (0: the
+    // whole-mesh HBM tables are used)
+    double wholePatchRadius = 0;
     // configuration
     Potential pot{POT_NONE, 0, 0, 0};
     double range = 1.0, rMax = HUGE_VAL;
@@ -131,6 +134,11 @@ constexpr int kBlock = 256;
 inline int blocksFor(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 __global__ void kFillInt(int* p, int v, long n) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void kFillDouble(double* p, double v, long n) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
@@ -595,8 +603,15 @@ static csg_status sysScreen(csg_ctx* c, SystemState* S, GeoParams& G) {
     return CSG_OK;
 }
 
+// Whole-mesh mode solved over patches: a geodesic no longer than the truncation radius r stays within Euclidean
+// distance r of its source, so every face it crosses has a corner within r plus the longest edge, and the submesh
+// rule with that radius holds every such path; distances up to r are those of the whole mesh (pairs beyond r carry
+// no force or energy, and windows beyond r are not propagated in either form). No sentinel, as in whole-mesh mode.
+static bool wholeAsPatches(const csg_ctx* c, const SystemState* S) { return !c->submeshing && S->nGroups == 0 && S->wholePatchRadius > 0; }
+
 // parameters of the force computation of the current step (the kept lists when the screen is on)
 static GeoParams forceParams(csg_ctx* c, SystemState* S) {
+    const bool patches = c->submeshing || wholeAsPatches(c, S);
     GeoParams G{};
     G.mesh = c->dm;
     G.pface = S->face.as<int32_t>();
@@ -605,7 +620,7 @@ static GeoParams forceParams(csg_ctx* c, SystemState* S) {
     G.srcIdx = nullptr;
     G.toff = S->toff.as<int32_t>();
     G.tidx = S->cand.as<int32_t>();
-    G.patchRadius = c->submeshing ? S->rad.as<double>() : nullptr;
+    G.patchRadius = patches ? S->rad.as<double>() : nullptr;
     G.rMax = S->rMax;
     G.forces = S->force.as<double>();
     G.energy = S->energy.as<double>();
@@ -626,7 +641,7 @@ static GeoParams forceParams(csg_ctx* c, SystemState* S) {
         G.blockOrder = S->order.as<int32_t>();
         G.blockChunk = (S->N + kXcds - 1) / kXcds;
     }
-    if (c->submeshing) {
+    if (patches) {
         G.patchList = S->plist.as<int32_t>();
         G.patchCount = S->pcount.as<int32_t>();
         G.patchStride = kPatchListCap;
@@ -642,7 +657,10 @@ static csg_status sysForces(csg_ctx* c, SystemState* S) {
     if (s) return s;
     GeoParams G = forceParams(c, S);
     G.keep = nullptr;  // set by the screen once this step's flags exist
-    if (c->submeshing) {
+    if (c->submeshing || wholeAsPatches(c, S)) {
+        if (!c->submeshing)  // every source's patch radius: the truncation radius plus the longest edge
+            hipLaunchKernelGGL(kFillDouble, dim3(blocksFor(S->N)), dim3(kBlock), 0, c->stream, S->rad.as<double>(), S->wholePatchRadius,
+                               (long)S->N);
         {
             MarkScope mark(S, c->stream, MARK_PATCH);
             HIPCHECK(c, launchPatch(G, S->plist.as<int32_t>(), S->pcount.as<int32_t>(), kPatchListCap, c->stream));
@@ -982,6 +1000,11 @@ csg_status csg_system_configure(csg_ctx* c, const csg_potential* pot, double ran
     S->scratchBlocks = blocks;
     csg_status st = sizeLastTier(c, S, std::max(c->maximumDistance, range));
     if (st) return st;
+    S->wholePatchRadius = 0;
+    if (!c->submeshing) {  // whole-mesh mode: patches when the truncation radius is finite and they fit the LDS tiers
+        const double r = exactTruncation(S->pot, S->rMax, false, 0.0);
+        if (std::isfinite(r) && r > 0 && patchFaceBound(c, r + c->maxEdge) <= 1536) S->wholePatchRadius = r + c->maxEdge;
+    }
     S->configured = true;
     return CSG_OK;
 }

@@ -32,8 +32,10 @@ def _tie_sources(osys):
     return np.array([np.any(fl[off[i]:off[i + 1]] & 2) for i in range(osys.N)])
 
 
-# N = 8192 fills exactly two 4,096-entry tiles of the candidate-offset scan; 20000 ends in a partial tile
-@pytest.mark.parametrize("cfg,n", [(2, 2000), (3, 4000), (1, 64), (3, 20000), (2, 8192)])
+# N = 8192 fills exactly two 4,096-entry tiles of the candidate-offset scan; 20000 ends in a partial tile. Config 1
+# (whole mesh, truncated) is solved over patches of the truncation radius; at N = 256 its 255 targets per source
+# exceed the first LDS tables and take the larger tiers
+@pytest.mark.parametrize("cfg,n", [(2, 2000), (3, 4000), (1, 64), (1, 256), (3, 20000), (2, 8192)])
 def test_candidates_and_forces(ctx, cfg, n):
     w = workloads.config(cfg, n=n)
     print(f"case system forces config {cfg} N={n}", flush=True)
