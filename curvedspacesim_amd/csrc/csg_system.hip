@@ -39,7 +39,8 @@ struct SystemState {
     DevBuf face, bary, vel, force, mass, energy;
     DevBuf xyz, sxyz, key, cellCount, cellStart, cellFill, order;
     DevBuf cnt, off, toff, cand, maxd, rad;
-    DevBuf disp, tstatus, gstatus, partial, counters, ke, chain, pstat, plist, pcount, spill, grab, redo, tspill;
+    // redo: the step's work counters (the XCD chunks' counters, then the two tiers' list counts) and the tiers' lists
+    DevBuf disp, tstatus, gstatus, partial, counters, ke, chain, pstat, plist, pcount, spill, redo, tspill;
     DevBuf scratch;
     DevBuf scanTmp;  // tile sums and offsets of the tiled scan
     // learned screen: features and logits of the candidate pairs, kept-pair lists; recorded pair distances
@@ -116,7 +117,7 @@ void destroySystem(SystemState* s) {
     if (s->graph) hipGraphExecDestroy(s->graph);
     DevBuf* all[] = {&s->face, &s->bary, &s->vel, &s->force, &s->mass, &s->energy, &s->xyz, &s->sxyz, &s->key, &s->cellCount, &s->cellStart,
                      &s->cellFill, &s->order, &s->cnt, &s->off, &s->toff, &s->cand, &s->maxd, &s->rad, &s->disp, &s->tstatus,
-                     &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->spill, &s->grab, &s->redo, &s->tspill, &s->scratch, &s->scratchBig,
+                     &s->gstatus, &s->partial, &s->counters, &s->ke, &s->chain, &s->pstat, &s->plist, &s->pcount, &s->spill, &s->redo, &s->tspill, &s->scratch, &s->scratchBig,
                      &s->sfeat, &s->sparams, &s->pdist, &s->skeep,
                      &s->grp, &s->gpot, &s->gr2, &s->gmaxd, &s->grmax, &s->gsent, &s->grange};
     for (DevBuf* b : all) b->release();
@@ -398,14 +399,6 @@ __global__ void kClampOffsets(int N, const int* off, long cap, int* toff, int* c
     if (i == N) counters[CNT_PAIRS] = off[N];
 }
 
-__global__ void kCountReasons(int N, const int* status, int* counters) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    int s = status[i];
-    for (int b = 0; b < 6; ++b)
-        if (s & (1 << b)) atomicAdd(&counters[CNT_LDS_REASON + b], 1);
-}
-
 __global__ void kCountNonzero(int N, const int* v, int* counters, int which, bool negativeOnly) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -670,21 +663,26 @@ static csg_status sysForces(csg_ctx* c, SystemState* S) {
             csg_status s2 = sysScreen(c, S, G);
             if (s2) return s2;
         }
+        // the step's counters, cleared by one kernel: the XCD chunks' (dynamic distribution of the sources over the
+        // XCDs' chunks), then each tier list's counts; then the lists
+        int32_t* grab = S->redo.as<int32_t>();
+        int32_t* count1 = grab + kXcds;
+        int32_t* count2 = count1 + 1 + kTierBuckets;
+        int32_t* list1 = count2 + 1 + kTierBuckets;
+        int32_t* list2 = list1 + (size_t)kTierBuckets * S->N;
         {
             MarkScope mark(S, c->stream, MARK_GEODESIC_LDS);
-            G.grab = S->grab.as<int>();  // dynamic distribution over the XCDs' chunks; counters cleared by a kernel
-            hipLaunchKernelGGL(kFillInt, dim3(1), dim3(kBlock), 0, c->stream, G.grab, 0, (long)kXcds);
+            G.grab = grab;
+            hipLaunchKernelGGL(kFillInt, dim3(1), dim3(kBlock), 0, c->stream, grab, 0, (long)(kXcds + 2 * (1 + kTierBuckets)));
             HIPCHECK(c, launchGeodesicLds(G, S->ldsVariant, c->stream));
             G.grab = nullptr;
         }
-        hipLaunchKernelGGL(kCountReasons, dim3(blocksFor(S->N)), dim3(kBlock), 0, c->stream, S->N, S->gstatus.as<int>(), S->counters.as<int>());
         MarkScope mark(S, c->stream, MARK_GEODESIC_HBM);
-        // the sources the LDS variant could not hold: two tiers of larger LDS tables, then HBM tables for what remains
-        for (int tier = 1; tier <= 2; ++tier) {
-            hipLaunchKernelGGL(kFillInt, dim3(1), dim3(kBlock), 0, c->stream, S->redo.as<int>(), 0, (long)(1 + kTierBuckets));
-            HIPCHECK(c, launchGeodesicLdsLarge(G, tier, S->redo.as<int32_t>() + 1 + kTierBuckets, S->redo.as<int32_t>(), S->tspill.as<SpillRecord>(),
-                                               c->stream));
-        }
+        // the sources the LDS variant could not hold (counted by reason): two tiers of larger LDS tables, each handing
+        // on what it cannot hold either, then HBM tables for what remains
+        HIPCHECK(c, launchListOverflowed(G, list1, count1, S->counters.as<int>() + CNT_LDS_REASON, c->stream));
+        HIPCHECK(c, launchGeodesicLdsLarge(G, 1, list1, count1, list2, count2, S->tspill.as<SpillRecord>(), c->stream));
+        HIPCHECK(c, launchGeodesicLdsLarge(G, 2, list2, count2, nullptr, nullptr, S->tspill.as<SpillRecord>(), c->stream));
         HIPCHECK(c, launchGeodesicScratchRedo(G, S->layout, S->scratchBlocks, c->stream));
         if (S->bigBlocks) HIPCHECK(c, launchGeodesicScratchRedo(G, S->layoutBig, S->bigBlocks, c->stream));
     } else {
@@ -940,8 +938,7 @@ csg_status csg_system_init(csg_ctx* c, int32_t N, const int32_t* face, const dou
     HIPCHECK(c, S->plist.ensure(4 * (size_t)kPatchListCap * N));
     HIPCHECK(c, S->pcount.ensure(4 * (size_t)N));
     HIPCHECK(c, S->spill.ensure(sizeof(SpillRecord) * (size_t)kSpillCap * N));
-    HIPCHECK(c, S->grab.ensure(sizeof(int) * kXcds));
-    HIPCHECK(c, S->redo.ensure(sizeof(int) * ((size_t)kTierBuckets * N + 1 + kTierBuckets)));
+    HIPCHECK(c, S->redo.ensure(sizeof(int) * (kXcds + 2 * (1 + kTierBuckets) + 2 * (size_t)kTierBuckets * N)));
     HIPCHECK(c, S->tspill.ensure(kTierSpillBytes));
     HIPCHECK(c, S->partial.ensure(8 * (size_t)blocksFor(N)));
     HIPCHECK(c, S->counters.ensure(4 * CNT_N));

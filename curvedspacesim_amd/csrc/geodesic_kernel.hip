@@ -1303,6 +1303,21 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
     __syncthreads();
 }
 
+// Sources a table capacity stopped: the larger tiers take them
+constexpr int kTierReasons = OVF_FACES | OVF_VERTS | OVF_QUEUE | OVF_TARGETS;
+
+// Appends source i to a tier list, in the bucket of its last completed solve's duration (patchStats, ticks of
+// 100 MHz): bucket b holds durations of about 2^(b+10) ticks (10 us) and up, the last everything from 1.3 ms on.
+__device__ void appendBucketed(int32_t* list, int32_t* count, int n, const int* patchStats, int i) {
+    int b = 0;
+    if (patchStats) {
+        const int* ps = patchStats + (size_t)kPatchStats * i;
+        const unsigned t = (unsigned)max(ps[4], 0) + (unsigned)max(ps[5], 0) + (unsigned)max(ps[6], 0);
+        b = min(max((31 - __clz((int)(t | 1u))) - 10, 0), kTierBuckets - 1);
+    }
+    list[(size_t)b * n + atomicAdd(count + 1 + b, 1)] = i;
+}
+
 }  // namespace
 
 // Source of this workgroup. Workgroups are dispatched round-robin over the 8 XCDs (workgroup i to XCD i mod 8), each
@@ -1339,7 +1354,8 @@ __device__ int grabSource(const GeoParams& P, int x) {
 // share the ring's storage with the vertex ids during setup and with the force reduction afterwards; kEnd: the variant
 // that also reports end tangents.
 template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, int WGS, bool kEnd, bool kList, int FR = 0>
-__global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P, const int32_t* list, const int32_t* count) {
+__global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P, const int32_t* list, const int32_t* count, int32_t* next,
+                                                             int32_t* nextCount) {
     constexpr int kRingBytes = QC * (5 * 8 + 4 + 8);
     static_assert(HF * 6 + HV * 6 + VC * 4 + FC * 4 <= kRingBytes, "setup hashes, vertex ids and list heads live in the ring's storage");
     static_assert(QC >= 4 * kMaxWaves, "the epilogue's reduction uses the ring's storage");
@@ -1402,14 +1418,20 @@ __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P, const i
         // at list + b * nSrc), so that no workgroup ends with a long source while the others idle
         for (;;) {
             if (threadIdx.x == 0) {
-                int i = atomicAdd(const_cast<int32_t*>(count), 1), s = -1;
-                for (int b = kTierBuckets - 1; b >= 0; --b) {
-                    const int nb = count[1 + b];
-                    if (i < nb) {
-                        s = list[(size_t)b * P.nSrc + i];
-                        break;
+                // the counter is read before it is taken, so that workgroups that find the list used up (or empty)
+                // leave without an atomic on the one address all of them share
+                int total = 0, s = -1;
+                for (int b = 0; b < kTierBuckets; ++b) total += count[1 + b];
+                if (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < total) {
+                    int i = atomicAdd(const_cast<int32_t*>(count), 1);
+                    for (int b = kTierBuckets - 1; b >= 0; --b) {
+                        const int nb = count[1 + b];
+                        if (i < nb) {
+                            s = list[(size_t)b * P.nSrc + i];
+                            break;
+                        }
+                        i -= nb;
                     }
-                    i -= nb;
                 }
                 sh.next = s;
             }
@@ -1418,6 +1440,9 @@ __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P, const i
             __syncthreads();  // every lane has read sh.next before the source's setup writes the shared state
             if (src < 0) break;
             solveSource<(FC + NT - 1) / NT, false>(P, T, sh, src, FR > 0 ? front : nullptr, FR > 0 ? front + FR : nullptr, FR);
+            // a source these tables cannot hold either goes on to the next tier's list (sh.overflow is read after the
+            // source's last barrier, before the next source resets it)
+            if (next && threadIdx.x == 0 && (sh.overflow & kTierReasons)) appendBucketed(next, nextCount, P.nSrc, P.patchStats, src);
         }
         return;
     }
@@ -1590,42 +1615,49 @@ static void launchLdsVariant(const GeoParams& P, hipStream_t stream) {
     // the others (the surplus exits at once)
     const int grid = P.grab ? 2 * kXcds * ((P.nSrc + kXcds - 1) / kXcds) : gridForSources(P);
     if (P.pairEnd)
-        hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, true, false>), dim3(grid), dim3(NT), 0, stream, P, none, none);
+        hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, true, false>), dim3(grid), dim3(NT), 0, stream, P, none, none,
+                           nullptr, nullptr);
     else
-        hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, false, false>), dim3(grid), dim3(NT), 0, stream, P, none, none);
+        hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, false, false>), dim3(grid), dim3(NT), 0, stream, P, none, none,
+                           nullptr, nullptr);
 }
 
 // sources whose first solve overflowed a capacity of the LDS tables (not the step limit) into list[0 .. *count)
-// The sources flagged for a table capacity, by the duration of their last completed solve (patchStats, in ticks of
-// 100 MHz): bucket b holds durations of about 2^(b+10) ticks (10 us) and up, the last everything from 1.3 ms on.
-__global__ void listOverflowedKernel(int n, const int* status, const int* patchStats, int32_t* list, int32_t* count) {
+// The sources flagged for a table capacity, into the first tier's list; reasons[b] (nullable) counts the sources
+// whose status has bit b.
+__global__ void listOverflowedKernel(int n, const int* status, const int* patchStats, int32_t* list, int32_t* count, int* reasons) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (!(status[i] & (OVF_FACES | OVF_VERTS | OVF_QUEUE | OVF_TARGETS))) return;
-    int b = 0;
-    if (patchStats) {
-        const int* ps = patchStats + (size_t)kPatchStats * i;
-        const unsigned t = (unsigned)max(ps[4], 0) + (unsigned)max(ps[5], 0) + (unsigned)max(ps[6], 0);
-        b = min(max((31 - __clz((int)(t | 1u))) - 10, 0), kTierBuckets - 1);
-    }
-    list[(size_t)b * n + atomicAdd(count + 1 + b, 1)] = i;
+    const int s = status[i];
+    if (!s) return;
+    if (reasons)
+        for (int b = 0; b < 6; ++b)
+            if (s & (1 << b)) atomicAdd(&reasons[b], 1);
+    if (s & kTierReasons) appendBucketed(list, count, n, patchStats, i);
 }
 
 template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, int WGS, int FR>
-static void launchLdsList(const GeoParams& P, const int32_t* list, const int32_t* count, int grid, hipStream_t stream) {
+static void launchLdsList(const GeoParams& P, const int32_t* list, const int32_t* count, int32_t* next, int32_t* nextCount, int grid,
+                          hipStream_t stream) {
     if (P.pairEnd)
         hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, true, true, FR>), dim3(grid), dim3(NT), 0, stream, P, list,
-                           count);
+                           count, next, nextCount);
     else
         hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, false, true, FR>), dim3(grid), dim3(NT), 0, stream, P, list,
-                           count);
+                           count, next, nextCount);
 }
 
-hipError_t launchGeodesicLdsLarge(const GeoParams& Q, int tier, int32_t* list, int32_t* count, SpillRecord* spill, hipStream_t stream) {
+hipError_t launchListOverflowed(const GeoParams& P, int32_t* list, int32_t* count, int* reasons, hipStream_t stream) {
+    if (P.nSrc <= 0) return hipSuccess;
+    hipLaunchKernelGGL(listOverflowedKernel, dim3((P.nSrc + 255) / 256), dim3(256), 0, stream, P.nSrc, (const int*)P.status,
+                       (const int*)P.patchStats, list, count, reasons);
+    return hipGetLastError();
+}
+
+hipError_t launchGeodesicLdsLarge(const GeoParams& Q, int tier, const int32_t* list, int32_t* count, int32_t* next, int32_t* nextCount,
+                                  SpillRecord* spill, hipStream_t stream) {
     if (Q.nSrc <= 0) return hipSuccess;
     GeoParams P = Q;
-    hipLaunchKernelGGL(listOverflowedKernel, dim3((P.nSrc + 255) / 256), dim3(256), 0, stream, P.nSrc, (const int*)P.status,
-                       (const int*)P.patchStats, list, count);
     // CUs of the current device (cached per device)
     static int cusOf[64] = {};
     int dev = 0;
@@ -1640,10 +1672,10 @@ hipError_t launchGeodesicLdsLarge(const GeoParams& Q, int tier, int32_t* list, i
     P.spillPerBlock = 1;
     P.spillCap = (int)(kTierSpillBytes / sizeof(SpillRecord) / grid);
     if (tier == 1)  // two 256-lane workgroups per CU, 78 KB of LDS each: up to 1,024 faces, 640 vertices, 256 targets
-        launchLdsList<1024, 640, 576, 256, 2048, 1024, 256, 2, 0>(P, list, count, grid, stream);
+        launchLdsList<1024, 640, 576, 256, 2048, 1024, 256, 2, 0>(P, list, count, next, nextCount, grid, stream);
     else  // one 512-lane workgroup per CU, 117 KB of LDS: up to 1,536 faces, 896 vertices, 256 targets; it searches the
           // patch itself when the patch kernel could not list it (more faces than a list holds)
-        launchLdsList<1536, 896, 896, 256, 4096, 2048, 512, 1, 512>(P, list, count, grid, stream);
+        launchLdsList<1536, 896, 896, 256, 4096, 2048, 512, 1, 512>(P, list, count, next, nextCount, grid, stream);
     return hipGetLastError();
 }
 
