@@ -55,6 +55,28 @@ def test_candidates_and_forces(ctx, cfg, n):
     assert st["unsolved"] == 0 and st["cand_overflow"] == 0 and st["pairs"] == off[-1]
 
 
+def test_whole_mesh_distances_over_patches(ctx):
+    """Whole-mesh mode with a finite truncation radius is solved over patches of that radius plus the longest edge:
+    every pair within the radius gets the distance of the oracle's whole-mesh solve (no patch, no truncation), to
+    1e-9 relative; the pairs beyond it carry no force either way."""
+    w = workloads.config(1, n=256)
+    print("case whole-mesh distances over patches config 1 N=256", flush=True)
+    sysm = workloads.make_system(ctx, w)
+    sysm.record_pairs(True)
+    F, E = sysm.compute_forces()
+    off, idx = sysm.pairs()
+    d = sysm.pair_distances()
+    sysm.record_pairs(False)
+    om, osys = _oracle(w)
+    r = osys.compute_forces()
+    assert np.array_equal(off, osys.last["off"]) and np.array_equal(idx, osys.last["idx"])
+    within = r["dist"] < w.range
+    assert within.sum() > 500
+    np.testing.assert_allclose(d[within], r["dist"][within], rtol=1e-9, atol=0)
+    assert np.all(d[~within] >= w.range)  # beyond the radius: unreached, or a path no shorter than the whole mesh's
+    assert sysm.status()["unsolved"] == 0
+
+
 @pytest.mark.parametrize("cfg,n", [(2, 2000), (3, 4000)])
 def test_one_step_matches_oracle(ctx, cfg, n):
     w = workloads.config(cfg, n=n)
