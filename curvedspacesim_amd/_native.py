@@ -80,6 +80,8 @@ def csg():
     L.csg_system_sync.argtypes = [VP]
     L.csg_system_get.argtypes = [VP, VP, VP, VP, VP]
     L.csg_system_set_state.argtypes = [VP, VP, VP, VP, VP]
+    L.csg_system_snapshot.argtypes = [VP]
+    L.csg_system_snapshot_read.argtypes = [VP, VP, VP, VP, VP, VP]
     L.csg_system_thermostat.argtypes = [VP, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double), VP,
                                         C.POINTER(C.c_double)]
     L.csg_system_set_thermostat.argtypes = [VP, C.c_int32, C.c_double, C.c_double, VP, C.c_double]
@@ -363,6 +365,21 @@ class System:
         F = np.zeros((self.N, 3))
         self.ctx._check(self.L.csg_system_get(self.ctx.h, _ptr(f), _ptr(b), _ptr(v), _ptr(F)), "csg_system_get")
         return f, b, v, F
+
+    def snapshot(self):
+        """Queues a copy of the state behind the steps queued so far and returns at once; snapshot_read() waits for it
+        only. For trajectory writers (simpleModelDatabase::writeState, src/databases/simpleModelDatabase.cpp:63-120)
+        that keep the device busy: queue steps, snapshot, queue more steps, then write the snapshot while they run."""
+        self.ctx._check(self.L.csg_system_snapshot(self.ctx.h), "csg_system_snapshot")
+
+    def snapshot_read(self):
+        """The last snapshot as a dict: faceIndex, barycentricPosition, R3position, velocity, force."""
+        out = dict(faceIndex=np.zeros(self.N, np.int32), barycentricPosition=np.zeros((self.N, 3)),
+                   R3position=np.zeros((self.N, 3)), velocity=np.zeros((self.N, 3)), force=np.zeros((self.N, 3)))
+        self.ctx._check(self.L.csg_system_snapshot_read(self.ctx.h, *(_ptr(out[k]) for k in
+                                                                      ("faceIndex", "barycentricPosition", "R3position",
+                                                                       "velocity", "force"))), "csg_system_snapshot_read")
+        return out
 
     def set_state(self, face=None, bary=None, vel=None, forces=None):
         """simpleModelDatabase::readState counterpart: replaces the given fields of the device state."""

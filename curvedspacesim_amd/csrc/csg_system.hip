@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "../../include/csg.h"
@@ -90,6 +91,13 @@ struct SystemState {
         hipEvent_t a, b;
     };
     std::vector<Mark> marks;
+    // asynchronous snapshots: device positions, and a pinned host copy of faces, barycentrics, positions, velocities
+    // and forces written by copies queued on the context stream, with the event that follows them
+    DevBuf snapXyz;
+    void* snapHost = nullptr;
+    size_t snapBytes = 0;
+    hipEvent_t snapEvent = nullptr;
+    bool snapTaken = false;
 };
 
 enum MarkKind : int { MARK_NEIGHBORS = 0, MARK_GEODESIC_LDS = 1, MARK_GEODESIC_HBM = 2, MARK_TRANSPORT = 3, MARK_PATCH = 4, MARK_SCREEN = 5, MARK_N = 6 };
@@ -121,6 +129,9 @@ void destroySystem(SystemState* s) {
                      &s->sfeat, &s->sparams, &s->pdist, &s->skeep,
                      &s->grp, &s->gpot, &s->gr2, &s->gmaxd, &s->grmax, &s->gsent, &s->grange};
     for (DevBuf* b : all) b->release();
+    s->snapXyz.release();
+    if (s->snapHost) hipHostFree(s->snapHost);
+    if (s->snapEvent) hipEventDestroy(s->snapEvent);
     for (auto& m : s->marks) {
         hipEventDestroy(m.a);
         hipEventDestroy(m.b);
@@ -906,6 +917,8 @@ csg_status csg_system_init(csg_ctx* c, int32_t N, const int32_t* face, const dou
     HIPCHECK(c, hipSetDevice(c->device));
     if (!c->sys) c->sys = new SystemState();
     SystemState* S = c->sys;
+    if (S->snapTaken) HIPCHECK(c, hipEventSynchronize(S->snapEvent));  // a snapshot of the previous system is dropped
+    S->snapTaken = false;
     S->N = N;
     S->configured = false;
     S->chainReady = false;
@@ -1106,6 +1119,52 @@ csg_status csg_system_get(csg_ctx* c, int32_t* face, double* bary, double* vel, 
     if (vel) HIPCHECK(c, hipMemcpyAsync(vel, S->vel.p, 24 * (size_t)S->N, hipMemcpyDeviceToHost, c->stream));
     if (forces) HIPCHECK(c, hipMemcpyAsync(forces, S->force.p, 24 * (size_t)S->N, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return CSG_OK;
+}
+
+csg_status csg_system_snapshot(csg_ctx* c) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S) return fail(c, CSG_ERR_STATE, "csg_system_init first");
+    HIPCHECK(c, hipSetDevice(c->device));
+    const size_t N = S->N, bytes = 4 * N + 4 * 24 * N;  // faces, then barycentrics, positions, velocities, forces
+    if (S->snapBytes < bytes) {
+        if (S->snapEvent && S->snapTaken) HIPCHECK(c, hipEventSynchronize(S->snapEvent));  // earlier copies still land here
+        if (S->snapHost) HIPCHECK(c, hipHostFree(S->snapHost));
+        S->snapHost = nullptr;
+        S->snapBytes = 0;
+        HIPCHECK(c, hipHostMalloc(&S->snapHost, bytes, hipHostMallocDefault));
+        S->snapBytes = bytes;
+    }
+    if (!S->snapEvent) HIPCHECK(c, hipEventCreateWithFlags(&S->snapEvent, hipEventDisableTiming));
+    HIPCHECK(c, S->snapXyz.ensure(24 * N));
+    char* h = (char*)S->snapHost;
+    hipLaunchKernelGGL(kPositions, dim3(blocksFor(S->N)), dim3(kBlock), 0, c->stream, c->dm, S->N, S->face.as<int32_t>(), S->bary.as<double>(),
+                       S->snapXyz.as<double>());
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipMemcpyAsync(h, S->face.p, 4 * N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(h + 4 * N, S->bary.p, 24 * N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(h + 28 * N, S->snapXyz.p, 24 * N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(h + 52 * N, S->vel.p, 24 * N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(h + 76 * N, S->force.p, 24 * N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipEventRecord(S->snapEvent, c->stream));
+    S->snapTaken = true;
+    return CSG_OK;
+}
+
+csg_status csg_system_snapshot_read(csg_ctx* c, int32_t* face, double* bary, double* xyz, double* vel, double* forces) {
+    SystemState* S = sysOf(c);
+    if (!c) return CSG_ERR_ARGUMENT;
+    if (!S || !S->snapTaken) return fail(c, CSG_ERR_STATE, "csg_system_snapshot first");
+    HIPCHECK(c, hipSetDevice(c->device));
+    HIPCHECK(c, hipEventSynchronize(S->snapEvent));  // the snapshot's copies only, not the steps queued after them
+    const size_t N = S->N;
+    const char* h = (const char*)S->snapHost;
+    if (face) std::memcpy(face, h, 4 * N);
+    if (bary) std::memcpy(bary, h + 4 * N, 24 * N);
+    if (xyz) std::memcpy(xyz, h + 28 * N, 24 * N);
+    if (vel) std::memcpy(vel, h + 52 * N, 24 * N);
+    if (forces) std::memcpy(forces, h + 76 * N, 24 * N);
     return CSG_OK;
 }
 

@@ -92,6 +92,12 @@ class TrajectoryDatabase:
         if self.L.csgh5_write(self.h, float(time), _ptr(f), _ptr(b), _ptr(p), _ptr(v), _ptr(F), _ptr(t)):
             self._fail("writing a record")
 
+    def write_snapshot(self, time, snap, types=None):
+        """Appends a record from System.snapshot_read() (or System.checkpoint()): velocities and forces as the file
+        saves them."""
+        self.write(time, snap["faceIndex"], snap["barycentricPosition"], snap["R3position"],
+                   snap["velocity"] if self.velocity else None, snap["force"] if self.force else None, types)
+
     def read(self, record=-1):
         """Record `record` (the last for -1) as a dict of the quantities the file saves (readState, plus R3position)."""
         N = self.N

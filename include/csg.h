@@ -159,6 +159,15 @@ csg_status csg_system_nvt(csg_ctx* ctx, double dt, double T, double tau, int32_t
 csg_status csg_system_sync(csg_ctx* ctx);
 /* copies state back (every pointer nullable): faces N, barycentric 3N, velocities 3N, forces 3N */
 csg_status csg_system_get(csg_ctx* ctx, int32_t* face, double* bary, double* vel, double* forces);
+/* asynchronous snapshot for trajectory writers (the state simpleModelDatabase::writeState records,
+ * src/databases/simpleModelDatabase.cpp:63-120, taken without stalling the steps): csg_system_snapshot queues, behind
+ * the steps queued so far, the Euclidean positions' computation and the copies of faces, barycentrics, positions,
+ * velocities and forces into a pinned host buffer the system owns, and returns at once; steps queued afterwards run
+ * while the host waits. csg_system_snapshot_read waits for those copies only and copies the snapshot into the caller's
+ * arrays (every pointer nullable: faces N, barycentric 3N, positions 3N, velocities 3N, forces 3N). One snapshot is
+ * held at a time: a later csg_system_snapshot replaces it. */
+csg_status csg_system_snapshot(csg_ctx* ctx);
+csg_status csg_system_snapshot_read(csg_ctx* ctx, int32_t* face, double* bary, double* xyz, double* vel, double* forces);
 /* checkpoint and resume. simpleModelDatabase::readState (src/databases/simpleModelDatabase.cpp:122-158) restores
  * positions, velocities and forces: csg_system_set_state does the same from host arrays (every pointer nullable: that
  * field is kept). The reference saves no thermostat state, so a resumed Nose-Hoover run restarts its chain; here the

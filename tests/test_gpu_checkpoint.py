@@ -84,3 +84,45 @@ def test_chainless_checkpoint_clears_the_thermostat():
     finally:
         ca.close()
         cb.close()
+
+
+def test_snapshot_overlaps_later_steps(tmp_path):
+    """csg_system_snapshot queues the state's copy behind the steps queued so far: steps queued after it do not change
+    what csg_system_snapshot_read returns, which equals the state at that point bit for bit (positions: the oracle's
+    barycentric interpolation, to rounding). A later snapshot replaces it; reading before any snapshot is an error."""
+    w = workloads.config(2, n=2000)
+    print("case asynchronous snapshot", flush=True)
+    c = csa.Context(0)
+    try:
+        s = workloads.make_system(c, w)
+        with pytest.raises(csa.CsgError):
+            s.snapshot_read()
+        s.nvt(w.dt, 0.5, 1.0, 2, 3)
+        fa, ba, va, Fa = s.get()
+        s.snapshot()
+        s.nvt(w.dt, 0.5, 1.0, 2, 4)
+        snap = s.snapshot_read()
+        np.testing.assert_array_equal(snap["faceIndex"], fa)
+        np.testing.assert_array_equal(snap["barycentricPosition"], ba)
+        np.testing.assert_array_equal(snap["velocity"], va)
+        np.testing.assert_array_equal(snap["force"], Fa)
+        om = OracleMesh(w.X, w.T)
+        np.testing.assert_allclose(snap["R3position"], om.positions(fa, ba), atol=1e-12)
+        fb, bb, vb, Fb = s.get()
+        assert not np.array_equal(vb, va)
+        s.snapshot()
+        again = s.snapshot_read()
+        np.testing.assert_array_equal(again["faceIndex"], fb)
+        np.testing.assert_array_equal(again["barycentricPosition"], bb)
+        np.testing.assert_array_equal(again["velocity"], vb)
+        if trajectory.available():
+            with trajectory.TrajectoryDatabase(tmp_path / "traj.h5", len(fa), trajectory.REPLACE, True, False, True) as db:
+                db.write_snapshot(3 * w.dt, snap)
+                db.write_snapshot(7 * w.dt, again)
+                r0, r1 = db.read(0), db.read(1)
+            np.testing.assert_array_equal(r0["faceIndex"], fa)
+            np.testing.assert_array_equal(r0["R3position"], snap["R3position"])
+            np.testing.assert_array_equal(r1["velocity"], vb)
+            assert r1["time"][0] == 7 * w.dt
+    finally:
+        c.close()
