@@ -262,17 +262,20 @@ csg_status preprocessMesh(csg_ctx* c, int nV, const double* X, int nF, const int
     HIPCHECK(c, upload(c, c->vflag, flag.data(), flag.size()));
     HIPCHECK(c, upload(c, c->vfoff, vfoff.data(), vfoff.size()));
     HIPCHECK(c, upload(c, c->vflist, vflist.data(), vflist.size()));
-    std::vector<double> corn(9 * (size_t)nF), nbv(9 * (size_t)nF, 0.0);
-    for (int f = 0; f < nF; ++f)
+    std::vector<double> rule(20 * (size_t)nF, 0.0);
+    for (int f = 0; f < nF; ++f) {
+        double* r = rule.data() + 20 * (size_t)f;
+        const int32_t words[4] = {adj[3 * f], adj[3 * f + 1], adj[3 * f + 2], 0};
+        std::memcpy(r, words, sizeof(words));
         for (int k = 0; k < 3; ++k) {
-            for (int a = 0; a < 3; ++a) corn[9 * f + 3 * k + a] = X[3 * F[3 * f + k] + a];
+            for (int a = 0; a < 3; ++a) r[2 + 3 * k + a] = X[3 * F[3 * f + k] + a];
             const int n = adj[3 * f + k];
             if (n < 0) continue;
             const int g = n >> 2, opp = F[3 * g + ((n & 3) + 2) % 3];  // the neighbour's corner off the shared edge
-            for (int a = 0; a < 3; ++a) nbv[9 * f + 3 * k + a] = X[3 * opp + a];
+            for (int a = 0; a < 3; ++a) r[11 + 3 * k + a] = X[3 * opp + a];
         }
-    HIPCHECK(c, upload(c, c->fcorn, corn.data(), corn.size()));
-    HIPCHECK(c, upload(c, c->fnbv, nbv.data(), nbv.size()));
+    }
+    HIPCHECK(c, upload(c, c->frule, rule.data(), rule.size()));
     std::vector<double> curv = vertexCurvatures(nV, X, nF, F, angle, flag);
     HIPCHECK(c, upload(c, c->curv, curv.data(), curv.size()));
     HIPCHECK(c, c->fchild.ensure(18 * sizeof(double) * (size_t)nF));
@@ -280,7 +283,7 @@ csg_status preprocessMesh(csg_ctx* c, int nV, const double* X, int nF, const int
     c->nF = nF;
     c->dm = csg::DeviceMesh{nV, nF, c->vx.as<double>(), c->fv.as<int32_t>(), c->fadj.as<int32_t>(), c->fgeo.as<double>(),
                             c->fnrm.as<double>(), c->vflag.as<uint8_t>(), c->vfoff.as<int32_t>(), c->vflist.as<int32_t>(),
-                            c->fchild.as<double>(), c->fcorn.as<double>(), c->fnbv.as<double>()};
+                            c->fchild.as<double>(), c->frule.as<double>()};
     HIPCHECK(c, csg::launchChildFrames(c->dm, c->fchild.as<double>(), c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     c->haveMesh = true;

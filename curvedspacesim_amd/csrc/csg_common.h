@@ -24,10 +24,10 @@ struct DeviceMesh {
     // {lL, uxL, uyL, lR, uxR, uyR}, the length and unit direction (in the edge-k frame) of corner k -> corner k+2 and
     // of corner k+2 -> corner k+1
     const double* __restrict__ fchild;
-    // 9 nF each, for the submesh search: a face's corner coordinates, and per local edge k the coordinates of the
-    // neighbour's corner opposite the shared edge (zero on a border edge)
-    const double* __restrict__ fcorn;
-    const double* __restrict__ fnbv;
+    // 20 nF, for the submesh search: per face a 160-byte record of its three adjacency words (and a pad word) in the
+    // first 16 bytes, its corner coordinates (9 doubles) and, per local edge k, the coordinates of the neighbour's corner
+    // opposite the shared edge (9 doubles, zero on a border edge). One record is ten 16-byte loads.
+    const double* __restrict__ frule;
 };
 
 // pair potential, reference conventions (src/forces/harmonicRepulsion.cpp, src/forces/gaussianRepulsion.cpp)

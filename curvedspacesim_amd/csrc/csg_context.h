@@ -51,7 +51,7 @@ struct csg_ctx {
     // mesh
     int nV = 0, nF = 0;
     double area = 0, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    DevBuf vx, fv, fadj, fgeo, fnrm, vflag, vfoff, vflist, curv, fchild, fcorn, fnbv;
+    DevBuf vx, fv, fadj, fgeo, fnrm, vflag, vfoff, vflist, curv, fchild, frule;
     csg::DeviceMesh dm{};
     std::vector<float> faceCentroid;  // host copy, 3 per face
     double maxEdge = 0;               // longest edge of the mesh
@@ -73,7 +73,7 @@ struct csg_ctx {
     std::string noallocErr;
     ~csg_ctx() {
         if (sys) csg::destroySystem(sys);
-        DevBuf* all[] = {&vx, &fv, &fadj, &fgeo, &fnrm, &vflag, &vfoff, &vflist, &curv, &fchild, &fcorn, &fnbv, &sX, &sY, &sOut, &sPart, &sGrad, &sParams, &sM, &sV, &pface, &pbary, &srcIdx, &toff, &tidx, &prad,
+        DevBuf* all[] = {&vx, &fv, &fadj, &fgeo, &fnrm, &vflag, &vfoff, &vflist, &curv, &fchild, &frule, &sX, &sY, &sOut, &sPart, &sGrad, &sParams, &sM, &sV, &pface, &pbary, &srcIdx, &toff, &tidx, &prad,
                          &pdist, &pst, &pet, &pflags, &pforces, &penergy, &status, &list, &plist, &pcount, &spill, &tdisp, &tvecs, &tstat, &scratch};
         for (DevBuf* b : all) b->release();
         if (stream) hipStreamDestroy(stream);

@@ -133,19 +133,7 @@ __device__ void blockExclusiveScan(IT* a, int n, Shared& sh) {
 }
 
 // Face hash. Lookups after a barrier that follows the last insertion see every key and slot; during an insertion
-// phase only the presence of a key (faceKnown) is reliable.
-template <class TT>
-__device__ bool faceKnown(const TT& T, int g) {
-    uint32_t h = hashOf((uint32_t)g) & T.hmask;
-    for (int p = 0; p <= T.hmask; ++p) {
-        int k = T.hkey[h];
-        if (k == g) return true;
-        if (k == -1) return false;
-        h = (h + 1) & T.hmask;
-    }
-    return false;
-}
-
+// phase only the presence of a key is reliable (faceInsert's own probe).
 template <class TT>
 __device__ int faceLookup(const TT& T, int g) {
     uint32_t h = hashOf((uint32_t)g) & T.hmask;
@@ -669,6 +657,63 @@ __device__ void initSourceFrame(const GeoParams& P, Shared& sh, int pi, int sf) 
     sh.t2[2] = n[0] * sh.t1[1] - n[1] * sh.t1[0];
 }
 
+// Whether point p lies within the radius of the source point S (squared distance not greater than r2). The sum is
+// written with explicit FMAs so that every search that applies the rule decides a corner the same way.
+__device__ inline bool withinRadius(const double* p, const double* S, double r2) {
+    const double dx = p[0] - S[0], dy = p[1] - S[1], dz = p[2] - S[2];
+    return !(fma(dz, dz, fma(dy, dy, dx * dx)) > r2);
+}
+
+// What the submesh rule needs of a face f: its adjacency words, its corners (c[0..9)) and, per local edge k, the far
+// corner of the neighbour across it (c[9 + 3k ..]), all from its 160-byte record in ten 16-byte loads. The neighbour
+// across edge k has f's corners k and k+1 and that far corner, so whether it has a corner within the radius is decided
+// from f's record alone.
+struct RuleFace {
+    int adj[3];
+    double c[18];
+};
+
+__device__ inline void loadRuleFace(const DeviceMesh& M, int f, RuleFace& F) {
+    const int4* rec = reinterpret_cast<const int4*>(M.frule + 20 * (size_t)f);
+    const int4 w = rec[0];
+    F.adj[0] = w.x;
+    F.adj[1] = w.y;
+    F.adj[2] = w.z;
+    const double2* d = reinterpret_cast<const double2*>(rec + 1);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const double2 v = d[i];
+        F.c[2 * i] = v.x;
+        F.c[2 * i + 1] = v.y;
+    }
+}
+
+// bit k: the neighbour across edge k of the face has a corner within the radius
+__device__ inline int ruleBits(const RuleFace& F, const double* S, double r2) {
+    bool in[3], inOpp[3];
+    for (int k = 0; k < 3; ++k) {
+        in[k] = withinRadius(F.c + 3 * k, S, r2);
+        inOpp[k] = withinRadius(F.c + 9 + 3 * k, S, r2);
+    }
+    int bits = 0;
+    for (int k = 0; k < 3; ++k)
+        if (in[k] || in[k == 2 ? 0 : k + 1] || inOpp[k]) bits |= 1 << k;
+    return bits;
+}
+
+// Whether the source's targets lie off its face (bit 0) and beyond its edge neighbours (bit 1), or'ed into sh.goals;
+// nbr: the neighbour faces of the source face (-2 across a border edge)
+__device__ inline void patchGoals(const GeoParams& P, Shared& sh, const int* nbr, int sf, int tb, int nT) {
+    int anyGoal = 0, beyond = 0;
+    for (int t = threadIdx.x; t < nT; t += blockDim.x) {
+        int f = P.pface[P.tidx[tb + t]];
+        if (f != sf) anyGoal = 1;
+        if (f != sf && f != nbr[0] && f != nbr[1] && f != nbr[2]) beyond = 1;
+    }
+    if (anyGoal) atomicOr(&sh.goals, 1);
+    if (beyond) atomicOr(&sh.goals, 2);
+}
+
 // The reference's submesh rule (src/utility/submesher.cpp:55-147): the source face; when some target lies elsewhere, its
 // edge neighbours; when some target lies beyond those, a breadth-first search over faces with at least one corner
 // within the radius of the source point (squared distance not greater than r^2) - the set reached equals the
@@ -682,14 +727,7 @@ __device__ void patchSearch(const GeoParams& P, const TT& T, Shared& sh, int* cu
     const double r2 = r * r;
     int nbr[3];
     for (int k = 0; k < 3; ++k) nbr[k] = M.fadj[3 * sf + k] < 0 ? -2 : (M.fadj[3 * sf + k] >> 2);
-    int anyGoal = 0, beyond = 0;
-    for (int t = lane; t < nT; t += blockDim.x) {
-        int f = P.pface[P.tidx[tb + t]];
-        if (f != sf) anyGoal = 1;
-        if (f != sf && f != nbr[0] && f != nbr[1] && f != nbr[2]) beyond = 1;
-    }
-    if (anyGoal) atomicOr(&sh.goals, 1);
-    if (beyond) atomicOr(&sh.goals, 2);
+    patchGoals(P, sh, nbr, sf, tb, nT);
     __syncthreads();
     if (lane == 0) {
         bool fresh;
@@ -711,27 +749,15 @@ __device__ void patchSearch(const GeoParams& P, const TT& T, Shared& sh, int* cu
             if (nc == 0) break;
             for (int i = lane; i < nc; i += blockDim.x) {
                 const int f = cur[i];
-                // the neighbour across edge k has corners k and k+1 of f and one more: one load level per face
-                int adj[3];
-                bool in[3], inOpp[3];
+                RuleFace F;
+                loadRuleFace(M, f, F);
+                const int bits = ruleBits(F, sh.S, r2);
                 for (int k = 0; k < 3; ++k) {
-                    adj[k] = M.fadj[3 * f + k];
-                    const double* p = M.fcorn + 9 * f + 3 * k;
-                    const double* q = M.fnbv + 9 * f + 3 * k;
-                    double dx = p[0] - sh.S[0], dy = p[1] - sh.S[1], dz = p[2] - sh.S[2];
-                    in[k] = !(dx * dx + dy * dy + dz * dz > r2);
-                    dx = q[0] - sh.S[0];
-                    dy = q[1] - sh.S[1];
-                    dz = q[2] - sh.S[2];
-                    inOpp[k] = !(dx * dx + dy * dy + dz * dz > r2);
-                }
-                for (int k = 0; k < 3; ++k) {
-                    int a = adj[k];
+                    int a = F.adj[k];
                     if (a < 0) continue;
                     int g = a >> 2;
-                    if (!(in[k] || in[k == 2 ? 0 : k + 1] || inOpp[k])) continue;  // every corner outside the radius
-                    if (faceKnown(T, g)) continue;
-                    bool fresh;
+                    if (!((bits >> k) & 1)) continue;  // every corner outside the radius
+                    bool fresh;  // (faceInsert finds a face already entered without a separate lookup)
                     faceInsert(T, sh, g, fresh);
                     if (fresh) {
                         int pos = atomicAdd(&sh.nNext, 1);

@@ -104,6 +104,40 @@ def test_one_step_matches_oracle(ctx, cfg, n):
         assert abs(st["kinetic"] - osys.KE) < 1e-6 * osys.KE
 
 
+@pytest.mark.parametrize("cfg,n", [(2, 2000), (3, 4000)])
+def test_forces_at_an_evolved_state(ctx, cfg, n):
+    """Forces after MD steps (particles moved off their initial faces, the step buffers reused from step to step): the
+    patches hold as many faces and vertices, summed over the sources, as those of a system set up in a new context
+    from the same state, and candidates and forces match the oracle's at that state."""
+    w = workloads.config(cfg, n=n)
+    print(f"case forces at an evolved state config {cfg} N={n}", flush=True)
+    sysm = workloads.make_system(ctx, w)
+    sysm.nvt(w.dt, w.temperature, w.tau, w.chain, 6)
+    f, b, v, _ = sysm.get()
+    F, E = sysm.compute_forces()
+    off, idx = sysm.pairs()
+    reused = sysm.patch_stats()
+    c2 = csa.Context(0)
+    try:
+        fresh = workloads.make_system(c2, w)
+        fresh.set_state(f, b, v)
+        F2, E2 = fresh.compute_forces()
+        ps = fresh.patch_stats()
+    finally:
+        c2.close()
+    assert reused["faces"] == ps["faces"] and reused["verts"] == ps["verts"] and reused["max_faces"] == ps["max_faces"]
+    om = OracleMesh(w.X, w.T)
+    osys = OracleSystem(om, f, b, v, w.potential, w.range, w.cell_list, w.submeshing, w.maximum_distance, threads=16)
+    osys.compute_forces()
+    assert np.array_equal(off, osys.last["off"]) and np.array_equal(idx, osys.last["idx"])
+    tie = _tie_sources(osys)
+    scale = np.abs(osys.F).max()
+    for got in (F, F2):
+        bad = np.where(~tie & (np.abs(got - osys.F).max(1) > 1e-4 * scale))[0]
+        assert len(bad) == 0
+    assert sysm.status()["unsolved"] == 0
+
+
 def test_nvt_full_size_runs(ctx):
     """The N = 20,000 workload for 20 steps: no errors, and the chain's kinetic energy equals that of the final
     velocities (the last chain half-step scales both by the same factor, noseHooverNVT.cpp:46-58)."""
