@@ -189,29 +189,32 @@ __device__ void faceInsertAt(const TT& T, int g, int slot) {
     }
 }
 
+// inserts vertex v (if absent) and gives it the next compact slot; returns its hash entry, whose slot (vhslot) is
+// readable after the next barrier, or -1 past the vertex capacity (the overflow bit is then set)
 template <class TT>
-__device__ void vertInsert(const TT& T, Shared& sh, int v) {
+__device__ int vertInsert(const TT& T, Shared& sh, int v) {
     uint32_t h = hashOf((uint32_t)v) & T.vhmask;
     for (int p = 0; p <= T.vhmask; ++p) {
         int k = T.vhkey[h];
-        if (k == v) return;
+        if (k == v) return (int)h;
         if (k == -1) {
             int old = atomicCAS(&T.vhkey[h], -1, v);
             if (old == -1) {
                 int slot = atomicAdd(&sh.nVerts, 1);
                 if (slot >= T.vcap) {
                     atomicOr(&sh.overflow, OVF_VERTS);
-                    return;
+                    return -1;
                 }
                 T.vhslot[h] = slot;
                 T.vglob[slot] = v;
-                return;
+                return (int)h;
             }
-            if (old == v) return;
+            if (old == v) return (int)h;
         }
         h = (h + 1) & T.vhmask;
     }
     atomicOr(&sh.overflow, OVF_VERTS);
+    return -1;
 }
 
 template <class TT>
@@ -843,7 +846,7 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int* cur
                 for (int j = 0; j < kR; ++j)
                     if (gl[j] >= 0) {
                         faceInsertAt(T, gl[j], lane + j * nt);
-                        for (int k = 0; k < 3; ++k) vertInsert(T, sh, cv[j][k]);
+                        for (int k = 0; k < 3; ++k) cv[j][k] = vertInsert(T, sh, cv[j][k]);  // ids become hash entries
                     }
                 cached = true;
             } else {
@@ -871,10 +874,11 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int* cur
         if (uniformOverflow(sh)) return false;
     }
     const int nV = sh.nVerts;
-    // C. per face slot: corner and neighbour slots, corner counts per vertex; per target: face slot and weights
-    auto faceSlotTables = [&](int s, const int* fvg, const int* adj) {
+    // C. per face slot: corner and neighbour slots, corner counts per vertex; per target: face slot and weights.
+    // Corners are given as vertex ids, or (entries) as the hash entries their insertion returned.
+    auto faceSlotTables = [&](int s, const int* fvg, const int* adj, bool entries) {
         for (int k = 0; k < 3; ++k) {
-            const int v = vertLookup(T, fvg[k]);
+            const int v = entries ? (int)T.vhslot[fvg[k]] : vertLookup(T, fvg[k]);
             T.fcv[3 * s + k] = v;
             atomicAdd(&T.vfo[v], 1);
             const int a = adj[k];
@@ -886,11 +890,11 @@ __device__ bool buildPatch(const GeoParams& P, const TT& T, Shared& sh, int* cur
     if (cached) {
 #pragma unroll
         for (int j = 0; j < kR; ++j)
-            if (lane + j * nt < nF) faceSlotTables(lane + j * nt, cv[j], ca[j]);
+            if (lane + j * nt < nF) faceSlotTables(lane + j * nt, cv[j], ca[j], true);
     } else {
         for (int s = lane; s < nF; s += nt) {
             const int g = T.fglob[s];
-            faceSlotTables(s, M.fv + 3 * g, M.fadj + 3 * g);
+            faceSlotTables(s, M.fv + 3 * g, M.fadj + 3 * g, false);
         }
     }
     for (int t = lane; t < nT; t += blockDim.x) {
