@@ -1336,15 +1336,18 @@ __device__ void solveSource(const GeoParams& P, const TT& T, Shared& sh, int src
 // Sources a table capacity stopped: the larger tiers take them
 constexpr int kTierReasons = OVF_FACES | OVF_VERTS | OVF_QUEUE | OVF_TARGETS;
 
-// Appends source i to a tier list, in the bucket of its last completed solve's duration (patchStats, ticks of
-// 100 MHz): bucket b holds durations of about 2^(b+10) ticks (10 us) and up, the last everything from 1.3 ms on.
+// The tier-list bucket of source i, by its last completed solve's duration (patchStats, ticks of 100 MHz): bucket b
+// holds durations of about 2^(b+10) ticks (10 us) and up, the last everything from 1.3 ms on.
+__device__ inline int tierBucket(const int* patchStats, int i) {
+    if (!patchStats) return 0;
+    const int* ps = patchStats + (size_t)kPatchStats * i;
+    const unsigned t = (unsigned)max(ps[4], 0) + (unsigned)max(ps[5], 0) + (unsigned)max(ps[6], 0);
+    return min(max((31 - __clz((int)(t | 1u))) - 10, 0), kTierBuckets - 1);
+}
+
+// appends source i to a tier list in its bucket
 __device__ void appendBucketed(int32_t* list, int32_t* count, int n, const int* patchStats, int i) {
-    int b = 0;
-    if (patchStats) {
-        const int* ps = patchStats + (size_t)kPatchStats * i;
-        const unsigned t = (unsigned)max(ps[4], 0) + (unsigned)max(ps[5], 0) + (unsigned)max(ps[6], 0);
-        b = min(max((31 - __clz((int)(t | 1u))) - 10, 0), kTierBuckets - 1);
-    }
+    const int b = tierBucket(patchStats, i);
     list[(size_t)b * n + atomicAdd(count + 1 + b, 1)] = i;
 }
 
@@ -1542,6 +1545,22 @@ __global__ __launch_bounds__(NT) void geodesicForceScratch(GeoParams P, ScratchL
     T.tcap = L.tcap;
     GeoParams Q = P;
     Q.spill = nullptr;  // the scratch ring is sized for a whole source
+    if constexpr (NT == 64) {
+        if (redoOnly && !list) {
+            // the sources the earlier kernels left unsolved (nonzero status): each workgroup tests a chunk of 64
+            // statuses at once and solves the flagged ones in turn (the ballot is the same on every lane)
+            for (int c = blockIdx.x * 64; c < nList; c += gridDim.x * 64) {
+                const int i = c + (int)threadIdx.x;
+                unsigned long long m = __ballot(i < nList && P.status[i] != 0);
+                while (m) {
+                    const int j = __ffsll((long long)m) - 1;
+                    m &= m - 1;
+                    solveSource<0, false>(Q, T, sh, c + j, cur, nxt, L.fcap);
+                }
+            }
+            return;
+        }
+    }
     for (int i = blockIdx.x; i < nList; i += gridDim.x) {
         const int src = list ? list[i] : i;
         if (redoOnly && P.status[src] == 0) continue;  // solved by the LDS variant
@@ -1652,18 +1671,29 @@ static void launchLdsVariant(const GeoParams& P, hipStream_t stream) {
                            nullptr, nullptr);
 }
 
-// sources whose first solve overflowed a capacity of the LDS tables (not the step limit) into list[0 .. *count)
-// The sources flagged for a table capacity, into the first tier's list; reasons[b] (nullable) counts the sources
-// whose status has bit b.
+// The sources flagged for a table capacity (not the step limit), into the first tier's list, bucketed as
+// appendBucketed does; reasons[b] (nullable) counts the sources whose status has bit b. Each counter is updated once
+// per wavefront and value, so that thousands of flagged sources do not queue on a few device-wide addresses.
 __global__ void listOverflowedKernel(int n, const int* status, const int* patchStats, int32_t* list, int32_t* count, int* reasons) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int s = status[i];
-    if (!s) return;
+    const int s = i < n ? status[i] : 0;  // every lane takes part in the ballots
+    const int lane = threadIdx.x & 63;
     if (reasons)
-        for (int b = 0; b < 6; ++b)
-            if (s & (1 << b)) atomicAdd(&reasons[b], 1);
-    if (s & kTierReasons) appendBucketed(list, count, n, patchStats, i);
+        for (int b = 0; b < 6; ++b) {
+            const unsigned long long m = __ballot((s >> b) & 1);
+            if (lane == 0 && m) atomicAdd(&reasons[b], __popcll(m));
+        }
+    const int bucket = (s & kTierReasons) ? tierBucket(patchStats, i) : -1;
+    const unsigned long long below = (1ull << lane) - 1;
+    for (int b = 0; b < kTierBuckets; ++b) {
+        const unsigned long long m = __ballot(bucket == b);
+        if (!m) continue;
+        const int leader = __ffsll((long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(count + 1 + b, __popcll(m));
+        base = __shfl(base, leader, 64);
+        if (bucket == b) list[(size_t)b * n + base + __popcll(m & below)] = i;
+    }
 }
 
 template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, int WGS, int FR>
