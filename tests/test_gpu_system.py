@@ -138,6 +138,30 @@ def test_forces_at_an_evolved_state(ctx, cfg, n):
     assert sysm.status()["unsolved"] == 0
 
 
+def test_sources_beyond_every_lds_tier(ctx):
+    """Patches of about 2,000 faces (harmonic sigma = 0.6 on a bumpy L5 sphere) exceed the first LDS tables and both
+    larger tiers (1,024 and 1,536 faces); the HBM-table kernel solves them, each of its workgroups testing chunks of 64
+    statuses for the sources left unsolved. Candidates and forces match the oracle's and nothing is left unsolved."""
+    rng = np.random.default_rng(21)
+    X, T = csa.bumpy_sphere(5, 20, 0.15, 0.3, 5)
+    n, sigma = 300, 0.6
+    f, b = workloads.area_uniform(X, T, n, rng)
+    w = workloads.Workload("bumpy sphere L5, N=300, harmonic sigma=0.6", X, T, f, b, np.zeros((n, 3)),
+                           (csa.HARMONIC, 1.0 / sigma ** 2, sigma), sigma, True, True, sigma, sigma, "nve", 0.005, 0.1)
+    print("case sources beyond every LDS tier, bumpy sphere L5 N=300 sigma=0.6", flush=True)
+    sysm = workloads.make_system(ctx, w)
+    F, E = sysm.compute_forces()
+    off, idx = sysm.pairs()
+    assert sysm.patch_stats()["max_faces"] > 1536  # only the HBM tables hold such a patch
+    om, osys = _oracle(w)
+    osys.compute_forces()
+    assert np.array_equal(off, osys.last["off"]) and np.array_equal(idx, osys.last["idx"])
+    tie = _tie_sources(osys)
+    bad = np.where(~tie & (np.abs(F - osys.F).max(1) > 1e-4 * np.abs(osys.F).max()))[0]
+    assert len(bad) == 0
+    assert sysm.status()["unsolved"] == 0
+
+
 def test_nvt_full_size_runs(ctx):
     """The N = 20,000 workload for 20 steps: no errors, and the chain's kinetic energy equals that of the final
     velocities (the last chain half-step scales both by the same factor, noseHooverNVT.cpp:46-58)."""
