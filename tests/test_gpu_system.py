@@ -175,11 +175,13 @@ def test_nvt_full_size_runs(ctx):
     assert np.isfinite(ke) and abs(st["kinetic"] - ke) < 1e-9 * ke
 
 
-def test_lds_variants_and_autotune(ctx):
+@pytest.mark.parametrize("cfg,n", [(3, 4000), (2, 2000)])
+def test_lds_variants_and_autotune(ctx, cfg, n):
     """Every variant of the LDS kernel gives the same forces (to the sum order of a source's pairs and FP32 tangents of
-    tied windows), and the autotuner keeps one of them."""
-    w = workloads.config(3, n=4000)
-    print("case lds variants config 3 N=4000", flush=True)
+    tied windows), and the autotuner keeps one of them: on the sphere's patches of about 380 faces and on the torus's
+    of about 70."""
+    w = workloads.config(cfg, n=n)
+    print(f"case lds variants config {cfg} N={n}", flush=True)
     sysm = workloads.make_system(ctx, w)
     out = []
     for v in range(csa.LDS_VARIANTS):
