@@ -324,7 +324,7 @@ csg_status runGeodesics(csg_ctx* c, csg::GeoParams& G, int maxTargets, bool whol
             G.spillCap = csg::kSpillCap;
             HIPCHECK(c, csg::launchPatch(G, c->plist.as<int32_t>(), c->pcount.as<int32_t>(), G.patchStride, c->stream));
         }
-        HIPCHECK(c, csg::launchGeodesicLds(G, 0, c->stream));
+        HIPCHECK(c, csg::launchGeodesicLds(G, 0, nullptr, nullptr, nullptr, c->stream));
         std::vector<int> st(G.nSrc);
         HIPCHECK(c, hipMemcpyAsync(st.data(), G.status, sizeof(int) * G.nSrc, hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -552,7 +552,7 @@ csg_status csg_pair_forces_noalloc(csg_ctx* c, int32_t nP, const int32_t* face, 
     G.rMax = csg::exactTruncation(G.pot, rMax, useSentinel != 0, sentinelDistance);
     // the same kernels as csg_pair_forces: patches, the LDS tables, then HBM tables of a fixed size for what they left
     HIPCHECK(c, csg::launchPatch(G, plist, pcount, G.patchStride, c->stream));
-    HIPCHECK(c, csg::launchGeodesicLds(G, 0, c->stream));
+    HIPCHECK(c, csg::launchGeodesicLds(G, 0, nullptr, nullptr, nullptr, c->stream));
     csg::ScratchLayout L = o.tablesLayout;
     L.base = base + o.tables;
     HIPCHECK(c, csg::launchGeodesicScratchRedo(G, L, kNoAllocFallbackBlocks, c->stream));

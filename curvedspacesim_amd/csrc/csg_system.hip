@@ -685,13 +685,13 @@ static csg_status sysForces(csg_ctx* c, SystemState* S) {
             MarkScope mark(S, c->stream, MARK_GEODESIC_LDS);
             G.grab = grab;
             hipLaunchKernelGGL(kFillInt, dim3(1), dim3(kBlock), 0, c->stream, grab, 0, (long)(kXcds + 2 * (1 + kTierBuckets)));
-            HIPCHECK(c, launchGeodesicLds(G, S->ldsVariant, c->stream));
+            // the sources these tables cannot hold go on to the first tier's list, counted by reason
+            HIPCHECK(c, launchGeodesicLds(G, S->ldsVariant, list1, count1, S->counters.as<int>() + CNT_LDS_REASON, c->stream));
             G.grab = nullptr;
         }
         MarkScope mark(S, c->stream, MARK_GEODESIC_HBM);
-        // the sources the LDS variant could not hold (counted by reason): two tiers of larger LDS tables, each handing
-        // on what it cannot hold either, then HBM tables for what remains
-        HIPCHECK(c, launchListOverflowed(G, list1, count1, S->counters.as<int>() + CNT_LDS_REASON, c->stream));
+        // the sources the LDS variant could not hold: two tiers of larger LDS tables, each handing on what it cannot
+        // hold either, then HBM tables for what remains
         HIPCHECK(c, launchGeodesicLdsLarge(G, 1, list1, count1, list2, count2, S->tspill.as<SpillRecord>(), c->stream));
         HIPCHECK(c, launchGeodesicLdsLarge(G, 2, list2, count2, nullptr, nullptr, S->tspill.as<SpillRecord>(), c->stream));
         HIPCHECK(c, launchGeodesicScratchRedo(G, S->layout, S->scratchBlocks, c->stream));

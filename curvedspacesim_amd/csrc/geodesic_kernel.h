@@ -98,17 +98,17 @@ hipError_t launchPatch(const GeoParams& P, int32_t* list, int32_t* count, int st
 // and 224 targets; 3 = 256 lanes, four workgroups per CU (40 KB each) with 576 faces, 320 vertices, a 280-record ring
 // and 128 targets
 constexpr int kLdsVariants = CSG_LDS_VARIANTS;
-hipError_t launchGeodesicLds(const GeoParams& P, int variant, hipStream_t stream);
 // The sources flagged for a table capacity (P.status), solved again with larger LDS tables, persistent over a list:
 // tier 1 up to 1,024 faces (two workgroups per CU), tier 2 up to 1,536 faces (one per CU, searching the patch itself
 // when the patch kernel could not list it), the longest first by their last solve's duration (P.patchStats, when
 // given). A list is kTierBuckets * nSrc ints with 1 + kTierBuckets counts, zero before it is built: the counter that
-// hands the entries out, then each bucket's length. launchListOverflowed builds the first tier's list from P.status
-// (and counts each status bit into reasons[0..6), nullable); each tier appends the sources it cannot hold either to
-// next/nextCount (nullable). spill: kTierSpillBytes, shared by the workgroups (P.spill is not used).
+// hands the entries out, then each bucket's length. launchGeodesicLds appends the sources its tables cannot hold to
+// the first tier's list next/nextCount (nullable), and counts each bit of their status into reasons[0..6) (nullable);
+// each tier appends the sources it cannot hold either to its next/nextCount (nullable). spill: kTierSpillBytes, shared
+// by the workgroups (P.spill is not used).
 constexpr int kTierBuckets = 8;
 constexpr size_t kTierSpillBytes = (size_t)512 * 8192 * sizeof(SpillRecord);
-hipError_t launchListOverflowed(const GeoParams& P, int32_t* list, int32_t* count, int* reasons, hipStream_t stream);
+hipError_t launchGeodesicLds(const GeoParams& P, int variant, int32_t* next, int32_t* nextCount, int* reasons, hipStream_t stream);
 hipError_t launchGeodesicLdsLarge(const GeoParams& P, int tier, const int32_t* list, int32_t* count, int32_t* next, int32_t* nextCount,
                                   SpillRecord* spill, hipStream_t stream);
 // fills mesh.fchild (out, 18 per face) from mesh.fgeo

@@ -1388,7 +1388,7 @@ __device__ int grabSource(const GeoParams& P, int x) {
 // that also reports end tangents.
 template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, int WGS, bool kEnd, bool kList, int FR = 0>
 __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P, const int32_t* list, const int32_t* count, int32_t* next,
-                                                             int32_t* nextCount) {
+                                                             int32_t* nextCount, int* reasons) {
     constexpr int kRingBytes = QC * (5 * 8 + 4 + 8);
     static_assert(HF * 6 + HV * 6 + VC * 4 + FC * 4 <= kRingBytes, "setup hashes, vertex ids and list heads live in the ring's storage");
     static_assert(QC >= 4 * kMaxWaves, "the epilogue's reduction uses the ring's storage");
@@ -1490,6 +1490,14 @@ __global__ __launch_bounds__(NT, WGS) void geodesicForceLds(GeoParams P, const i
     }
     if (src < 0) return;
     solveSource<(FC + NT - 1) / NT, true>(P, T, sh, src, nullptr, nullptr, 0);
+    // a source these tables could not hold goes on to the first tier's list (sh.overflow is read after the source's
+    // last barrier)
+    if (next && threadIdx.x == 0 && sh.overflow) {
+        if (reasons)
+            for (int b = 0; b < 6; ++b)
+                if (sh.overflow & (1 << b)) atomicAdd(&reasons[b], 1);
+        if (sh.overflow & kTierReasons) appendBucketed(next, nextCount, P.nSrc, P.patchStats, src);
+    }
 }
 
 // Scratch variant: the same solver with its tables in a per-workgroup slice of an HBM buffer, persistent over a
@@ -1658,42 +1666,17 @@ ScratchLayout makeScratchLayout(long faces, long targets) {
 }
 
 template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, int WGS>
-static void launchLdsVariant(const GeoParams& P, hipStream_t stream) {
+static void launchLdsVariant(const GeoParams& P, int32_t* next, int32_t* nextCount, int* reasons, hipStream_t stream) {
     const int32_t* none = nullptr;
     // dynamic distribution: twice the sources' workgroups, so that an XCD whose chunk costs less can take sources from
     // the others (the surplus exits at once)
     const int grid = P.grab ? 2 * kXcds * ((P.nSrc + kXcds - 1) / kXcds) : gridForSources(P);
     if (P.pairEnd)
         hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, true, false>), dim3(grid), dim3(NT), 0, stream, P, none, none,
-                           nullptr, nullptr);
+                           next, nextCount, reasons);
     else
         hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, false, false>), dim3(grid), dim3(NT), 0, stream, P, none, none,
-                           nullptr, nullptr);
-}
-
-// The sources flagged for a table capacity (not the step limit), into the first tier's list, bucketed as
-// appendBucketed does; reasons[b] (nullable) counts the sources whose status has bit b. Each counter is updated once
-// per wavefront and value, so that thousands of flagged sources do not queue on a few device-wide addresses.
-__global__ void listOverflowedKernel(int n, const int* status, const int* patchStats, int32_t* list, int32_t* count, int* reasons) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = i < n ? status[i] : 0;  // every lane takes part in the ballots
-    const int lane = threadIdx.x & 63;
-    if (reasons)
-        for (int b = 0; b < 6; ++b) {
-            const unsigned long long m = __ballot((s >> b) & 1);
-            if (lane == 0 && m) atomicAdd(&reasons[b], __popcll(m));
-        }
-    const int bucket = (s & kTierReasons) ? tierBucket(patchStats, i) : -1;
-    const unsigned long long below = (1ull << lane) - 1;
-    for (int b = 0; b < kTierBuckets; ++b) {
-        const unsigned long long m = __ballot(bucket == b);
-        if (!m) continue;
-        const int leader = __ffsll((long long)m) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(count + 1 + b, __popcll(m));
-        base = __shfl(base, leader, 64);
-        if (bucket == b) list[(size_t)b * n + base + __popcll(m & below)] = i;
-    }
+                           next, nextCount, reasons);
 }
 
 template <int FC, int VC, int QC, int TC, int HF, int HV, int NT, int WGS, int FR>
@@ -1701,17 +1684,10 @@ static void launchLdsList(const GeoParams& P, const int32_t* list, const int32_t
                           hipStream_t stream) {
     if (P.pairEnd)
         hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, true, true, FR>), dim3(grid), dim3(NT), 0, stream, P, list,
-                           count, next, nextCount);
+                           count, next, nextCount, nullptr);
     else
         hipLaunchKernelGGL((geodesicForceLds<FC, VC, QC, TC, HF, HV, NT, WGS, false, true, FR>), dim3(grid), dim3(NT), 0, stream, P, list,
-                           count, next, nextCount);
-}
-
-hipError_t launchListOverflowed(const GeoParams& P, int32_t* list, int32_t* count, int* reasons, hipStream_t stream) {
-    if (P.nSrc <= 0) return hipSuccess;
-    hipLaunchKernelGGL(listOverflowedKernel, dim3((P.nSrc + 255) / 256), dim3(256), 0, stream, P.nSrc, (const int*)P.status,
-                       (const int*)P.patchStats, list, count, reasons);
-    return hipGetLastError();
+                           count, next, nextCount, nullptr);
 }
 
 hipError_t launchGeodesicLdsLarge(const GeoParams& Q, int tier, const int32_t* list, int32_t* count, int32_t* next, int32_t* nextCount,
@@ -1739,16 +1715,18 @@ hipError_t launchGeodesicLdsLarge(const GeoParams& Q, int tier, const int32_t* l
     return hipGetLastError();
 }
 
-hipError_t launchGeodesicLds(const GeoParams& P, int variant, hipStream_t stream) {
+hipError_t launchGeodesicLds(const GeoParams& P, int variant, int32_t* next, int32_t* nextCount, int* reasons, hipStream_t stream) {
     if (P.nSrc <= 0) return hipSuccess;
     if (variant == 1)
-        launchLdsVariant<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsFaceHash, kLdsVertHash, 512, 2>(P, stream);
+        launchLdsVariant<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsFaceHash, kLdsVertHash, 512, 2>(P, next, nextCount,
+                                                                                                                  reasons, stream);
     else if (variant == 2)  // smaller tables and ring, three workgroups per CU
-        launchLdsVariant<576, 352, 384, 224, 1024, 512, 256, 3>(P, stream);
+        launchLdsVariant<576, 352, 384, 224, 1024, 512, 256, 3>(P, next, nextCount, reasons, stream);
     else if (variant == 3)  // four workgroups per CU: 40 KB each
-        launchLdsVariant<576, 320, 280, 128, 1024, 512, 256, 4>(P, stream);
+        launchLdsVariant<576, 320, 280, 128, 1024, 512, 256, 4>(P, next, nextCount, reasons, stream);
     else
-        launchLdsVariant<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsFaceHash, kLdsVertHash, 256, 2>(P, stream);
+        launchLdsVariant<kLdsFaceCap, kLdsVertCap, kLdsQueueCap, kLdsTargetCap, kLdsFaceHash, kLdsVertHash, 256, 2>(P, next, nextCount,
+                                                                                                                  reasons, stream);
     return hipGetLastError();
 }
 
