@@ -22,6 +22,26 @@ def _state(rng, n):
                 vel=rng.normal(size=(n, 3)), force=rng.normal(size=(n, 3)), types=rng.integers(0, 3, n).astype(np.int32))
 
 
+def test_write_snapshot_saves_what_the_file_keeps(tmp_path, rng):
+    """write_snapshot takes the dict of System.snapshot_read(): velocities and forces go in when the file saves them
+    and are left out when it does not."""
+    n = 23
+    s = _state(rng, n)
+    snap = dict(faceIndex=s["face"], barycentricPosition=s["bary"], R3position=s["r3"], velocity=s["vel"], force=s["force"])
+    with trajectory.TrajectoryDatabase(tmp_path / "all.h5", n, trajectory.REPLACE, True, False, True) as db:
+        db.write_snapshot(0.25, snap)
+        r = db.read(0)
+    assert r["time"][0] == 0.25
+    for k, v in (("faceIndex", s["face"]), ("barycentricPosition", s["bary"]), ("R3position", s["r3"]), ("velocity", s["vel"]),
+                 ("force", s["force"])):
+        np.testing.assert_array_equal(r[k], v)
+    with trajectory.TrajectoryDatabase(tmp_path / "positions.h5", n, trajectory.REPLACE, False, False, False) as db:
+        db.write_snapshot(0.5, snap)
+        r = db.read(0)
+    assert "velocity" not in r and "force" not in r
+    np.testing.assert_array_equal(r["R3position"], s["r3"])
+
+
 def test_round_trip_and_append(tmp_path, rng):
     path = tmp_path / "traj.h5"
     n = 37
